@@ -1,0 +1,192 @@
+/*
+ * sgpe.h - C ABI of libsgpe, the MI355X (gfx950) split-step engine for the
+ * quasi-2D two-component Gross-Pitaevskii equation.
+ *
+ * The reference (ultracoldYEG/spinor-gpe) is pure Python on PyTorch; this
+ * library replaces the torch calls below its TensorPropagator and
+ * tensor_tools layers. Each entry point names the reference code it stands
+ * in for (paths relative to the reference repository root). The Python
+ * package spinor_gpe binds these symbols with ctypes.
+ *
+ * Conventions
+ *  - Every int-returning function returns SGPE_OK (0) or a negative error
+ *    code; sgpe_last_error() then describes the failure (thread-local).
+ *    No C++ exception crosses this boundary.
+ *  - Device pointers are HIP device addresses (e.g. torch.Tensor.data_ptr()
+ *    of a tensor on device 'cuda'). Host pointers are only read during the
+ *    call.
+ *  - A state ("psik") is one contiguous [2][ny][nx] complex array
+ *    (complex128 or complex64, interleaved re/im), component 0 first, in the
+ *    reference's fft-shifted k-space layout (x along the fast axis).
+ *  - All work is enqueued on the context's stream and is asynchronous unless
+ *    stated otherwise.
+ */
+#ifndef SGPE_H
+#define SGPE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SGPE_ABI_VERSION 1
+
+enum {
+  SGPE_OK = 0,
+  SGPE_E_ARG = -1,         /* invalid argument */
+  SGPE_E_HIP = -2,         /* HIP runtime error */
+  SGPE_E_NOMEM = -3,       /* device allocation failed */
+  SGPE_E_UNSUPPORTED = -4  /* grid size / precision not supported */
+};
+
+enum { SGPE_C128 = 0, SGPE_C64 = 1 };            /* state precision */
+enum { SGPE_TIME_IMAG = 0, SGPE_TIME_REAL = 1 };  /* tensor_propagator.py:96-99 */
+enum { SGPE_COUPLING_OFF = 0, SGPE_COUPLING_UNIFORM = 1, SGPE_COUPLING_ARRAY = 2 };
+enum { SGPE_SUB_OUTER = 0, SGPE_SUB_INNER = 1 };  /* tensor_propagator.py:101-103 */
+
+typedef struct sgpe_ctx sgpe_ctx;
+typedef struct sgpe_plan sgpe_plan;
+
+/*
+ * Physical description of one instance: the inputs TensorPropagator.__init__
+ * reads from a PSpinor (tensor_propagator.py:91-129). All pointers are HOST
+ * float64 arrays, copied into the plan by sgpe_plan_set_slot.
+ *
+ * Kinetic energy T_s (kin_eng_spin) and potential V_s (pot_eng_spin) are each
+ * given either separably, T_s[jy][jx] = t_x[s][jx] + t_y[s][jy], or densely
+ * (t_dense[s], ny*nx row-major); a non-NULL dense pointer wins.
+ * theta_x: the coupling phase expon = 2 kL x (:126-129), a function of x only;
+ * NULL in the rotated frame (expon = 0).
+ */
+typedef struct sgpe_inst_desc {
+  int ny, nx;
+  int time_mode;           /* SGPE_TIME_* */
+  double t_step;           /* full-step duration dt > 0 */
+  double atom_num;
+  double g_uu, g_dd, g_ud; /* scaled interaction strengths (spin.g_sc) */
+  double dx, dy;           /* space['dr'] */
+  const double* t_x[2];
+  const double* t_y[2];
+  const double* t_dense[2];
+  const double* v_x[2];
+  const double* v_y[2];
+  const double* v_dense[2];
+  int coupling;            /* SGPE_COUPLING_*; spin.is_coupling */
+  double omega;            /* uniform coupling strength (SGPE_COUPLING_UNIFORM) */
+  const double* omega_dense; /* ny*nx coupling array (SGPE_COUPLING_ARRAY) */
+  const double* theta_x;   /* nx values or NULL */
+} sgpe_inst_desc;
+
+/* ABI version of the loaded library (compare with SGPE_ABI_VERSION). */
+int sgpe_abi_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* sgpe_last_error(void);
+
+/* ---------------------------------------------------------------- context */
+
+/* Bind a device and a stream (hipStream_t, NULL = default stream). Replaces
+ * the reference's implicit torch device/stream (tensor_propagator.py:92). */
+int sgpe_ctx_create(int device, void* stream, sgpe_ctx** out);
+int sgpe_ctx_set_stream(sgpe_ctx* ctx, void* stream);
+int sgpe_ctx_synchronize(sgpe_ctx* ctx);
+void sgpe_ctx_destroy(sgpe_ctx* ctx);
+
+/* ------------------------------------------------- tensor_tools primitives */
+
+/* fft_2d / ifft_2d (spinor_gpe/pspinor/tensor_tools.py:201-228, 231-258) on
+ * n_arrays device arrays of shape (ny, nx):
+ *   inverse = 0: out = fftshift(fftn(in)) * dx*dy/(2 pi)
+ *   inverse = 1: out = ifftn(ifftshift(in)) / (dx*dy/(2 pi))
+ * in may equal out. */
+int sgpe_fft2(sgpe_ctx* ctx, int precision, int ny, int nx, int n_arrays, void* const* in, void* const* out,
+              int inverse, double dx, double dy);
+
+/* norm (tensor_tools.py:261-311, pop_frac=None): normalize the two
+ * components to atom_num in place; optional dens0/dens1 (float64 for
+ * SGPE_C128, float32 for SGPE_C64) receive the normalized densities;
+ * nf_out (host, optional) receives the norm factor. Synchronizes the stream
+ * when nf_out is given. */
+int sgpe_norm(sgpe_ctx* ctx, int precision, int ny, int nx, void* psi0, void* psi1, double vol_elem,
+              double atom_num, void* dens0, void* dens1, double* nf_out);
+
+/* calc_pops (tensor_tools.py:466-484): out2[s] = vol_elem * sum |psi_s|^2,
+ * written to HOST memory (synchronizes the stream). */
+int sgpe_pops(sgpe_ctx* ctx, int precision, int ny, int nx, const void* psi0, const void* psi1, double vol_elem,
+              double* out2);
+
+/* --------------------------------------------------------- propagation */
+
+/* A plan of n_slots instances sharing one grid (ny, nx) and precision.
+ * Stands in for TensorPropagator's operator set-up (tensor_propagator.py:
+ * 67-149), batched. ny, nx: powers of two in [64, 4096]. */
+int sgpe_plan_create(sgpe_ctx* ctx, int precision, int ny, int nx, int n_slots, sgpe_plan** out);
+void sgpe_plan_destroy(sgpe_plan* plan);
+
+/* Load one slot's physics (host arrays are copied) and bind its device state
+ * psik ([2][ny][nx]); the slot becomes active with a k-space state and a
+ * fresh run (tensor_propagator.py:91-149). Synchronizes the stream. */
+int sgpe_plan_set_slot(sgpe_plan* plan, int slot, const sgpe_inst_desc* desc, void* psik);
+
+/* Rebind a slot to a (new) normalized k-space state psik and start a fresh
+ * run; physics tables are kept. Synchronizes the stream. */
+int sgpe_plan_set_state(sgpe_plan* plan, int slot, void* psik);
+
+/* Start a run for one slot: device buffer pops (2 doubles per full step,
+ * capacity cap steps; NULL for none) receives the per-step populations of
+ * prop_loop (tensor_propagator.py:173-194); conv_tol > 0 enables the
+ * energy-based convergence test of sgpe_check_convergence. */
+int sgpe_plan_set_run(sgpe_plan* plan, int slot, double* pops, int cap, double conv_tol);
+
+/* Enable (1) or disable (0) a slot; disabled slots are skipped by every
+ * kernel (their blocks exit at once). */
+int sgpe_plan_set_active(sgpe_plan* plan, int slot, int active);
+
+/* Select FFT shapes: elements per thread and rows per block for the row
+ * pass, elements per thread and columns per block for the column pass.
+ * 0 keeps the current value. Returns SGPE_E_UNSUPPORTED if not compiled. */
+int sgpe_plan_tune(sgpe_plan* plan, int row_e, int row_rows, int col_e, int col_w);
+
+/* Current shapes: out4 = {row_e, row_rows, col_e, col_w}. */
+int sgpe_plan_get_tuning(const sgpe_plan* plan, int* out4);
+
+/* single_step (tensor_propagator.py:224-271) for every active slot with
+ * the outer (SGPE_SUB_OUTER) or inner (SGPE_SUB_INNER) sub-step duration and
+ * operators. The closing kinetic half-step and renormalization are applied
+ * lazily (by the next step or by sgpe_plan_sync). */
+int sgpe_single_step(sgpe_plan* plan, int which);
+
+/* n_steps x full_step (tensor_propagator.py:214-222), including the per-step
+ * populations of the prop_loop body (:191-194) when a pops buffer is set. */
+int sgpe_full_steps(sgpe_plan* plan, int n_steps);
+
+/* Same as sgpe_full_steps(plan, steps_per_graph * n_graphs), issued as
+ * n_graphs launches of one captured hipGraph of steps_per_graph steps. */
+int sgpe_graph_steps(sgpe_plan* plan, int steps_per_graph, int n_graphs);
+
+/* Bring every active slot's state to normalized k-space psik (closing
+ * kinetic half-step + tensor_propagator.py:271) and record the last step's
+ * populations. */
+int sgpe_plan_sync(sgpe_plan* plan);
+
+/* GP energy functional of each active slot's current state, in float64:
+ * out[8*slot + i] (device buffer) = {E_tot, E_kin, E_pot, E_int, E_cpl, mu,
+ * N, rel_change} with E the total energy (not per atom) and mu the chemical
+ * potential. The reference computes its (different) end-of-run energy on the
+ * host in NumPy (tensor_propagator.py:273-324); this is the engine's own
+ * spectral evaluation, used for convergence and chemical potential. Brings
+ * the state to normalized k-space first (as sgpe_plan_sync). */
+int sgpe_energy(sgpe_plan* plan, double* out);
+
+/* ------------------------------------------------------------- host tools */
+
+/* 2D phase unwrapping by reliability-sorted path following (Herraez et al.,
+ * Appl. Opt. 41, 7437 (2002)), the published algorithm behind
+ * skimage.restoration.unwrap_phase used by tensor_tools.phase_comp
+ * (tensor_tools.py:528-531). Host arrays, ny*nx row-major, no wrap-around. */
+int sgpe_unwrap_phase_2d(const double* wrapped, double* out, int ny, int nx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SGPE_H */

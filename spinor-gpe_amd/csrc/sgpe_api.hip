@@ -1,0 +1,852 @@
+// Host side of libsgpe: contexts, plans, kernel registry, launch sequences,
+// hipGraph capture and the extern "C" entry points declared in sgpe.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/sgpe.h"
+#include "sgpe_kernels.hip"
+
+using namespace sgpe;
+
+// ------------------------------------------------------------------ errors
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                             \
+  do {                                                                                            \
+    hipError_t e_ = (expr);                                                                       \
+    if (e_ != hipSuccess)                                                                         \
+      return fail(SGPE_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));                  \
+  } while (0)
+
+// -------------------------------------------------------------- twiddles
+// tw[m] = exp(-2 pi i m / n), from libm cos/sin of angles reduced exactly to
+// [0, pi/4] by the symmetries of the unit circle (n is a power of two, so
+// m/n and the reductions are exact).
+static void twiddles_f64(int n, std::vector<double>& re, std::vector<double>& im) {
+  re.resize(n);
+  im.resize(n);
+  const int q = n / 4, o = n / 8;
+  for (int m = 0; m < n; ++m) {
+    const int quad = m / q, j = m % q;
+    double c, s;  // cos, sin of 2 pi j / n, j in [0, n/4)
+    if (j <= o) {
+      const double a = (2.0 * M_PI / n) * j;
+      c = std::cos(a);
+      s = std::sin(a);
+    } else {
+      const double a = (2.0 * M_PI / n) * (q - j);
+      c = std::sin(a);
+      s = std::cos(a);
+    }
+    // rotate by quad * pi/2: (c, s) -> (-s, c)
+    double cc = c, ss = s;
+    for (int r = 0; r < quad; ++r) {
+      const double t = cc;
+      cc = -ss;
+      ss = t;
+    }
+    re[m] = cc;
+    im[m] = -ss;  // exp(-i angle)
+  }
+}
+
+// ------------------------------------------------------- kernel registry
+
+typedef void (*ColFn)(dim3, dim3, size_t, hipStream_t, const InstDev*, RunDev*, const void*, int, int, int);
+typedef void (*RowFn)(dim3, dim3, size_t, hipStream_t, const InstDev*, RunDev*, const void*, int, int, int);
+typedef void (*FRowFn)(dim3, dim3, size_t, hipStream_t, const void*, void*, int, int, double, const void*);
+typedef void (*FColFn)(dim3, dim3, size_t, hipStream_t, const void*, void*, int, int, const void*);
+
+template <typename R, int N, int E, int W>
+static void col_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const InstDev* i, RunDev* r, const void* tw, int m,
+                       int dp, int dc) {
+  hipLaunchKernelGGL((col_pass_kernel<R, N, E, W>), g, b, l, s, i, r, (const typename Cx<R>::T*)tw, m, dp, dc);
+}
+template <typename R, int N, int E, int ROWS>
+static void row_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const InstDev* i, RunDev* r, const void* tw,
+                       int dur, int first, int colw) {
+  hipLaunchKernelGGL((row_pass_kernel<R, N, E, ROWS>), g, b, l, s, i, r, (const typename Cx<R>::T*)tw, dur, first,
+                     colw);
+}
+template <typename R, int N, int E, int ROWS>
+static void frow_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const void* in, void* out, int inv, int shift,
+                        double scale, const void* tw) {
+  typedef typename Cx<R>::T C;
+  hipLaunchKernelGGL((fft_rows_kernel<R, N, E, ROWS>), g, b, l, s, (const C*)in, (C*)out, inv, shift, scale,
+                     (const C*)tw);
+}
+template <typename R, int N, int E, int W>
+static void fcol_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const void* in, void* out, int nx, int inv,
+                        const void* tw) {
+  typedef typename Cx<R>::T C;
+  hipLaunchKernelGGL((fft_cols_kernel<R, N, E, W>), g, b, l, s, (const C*)in, (C*)out, nx, inv, (const C*)tw);
+}
+
+struct KEntry {
+  int prec, n, e, p;  // p = W (columns) or ROWS (rows)
+  void* launch;
+  const void* kernel;
+};
+
+#define COL_E(R, PR, N, E, W) {PR, N, E, W, (void*)&col_launch<R, N, E, W>, (const void*)&col_pass_kernel<R, N, E, W>},
+#define ROW_E(R, PR, N, E, RW) \
+  {PR, N, E, RW, (void*)&row_launch<R, N, E, RW>, (const void*)&row_pass_kernel<R, N, E, RW>},
+#define FROW_E(R, PR, N, E, RW) \
+  {PR, N, E, RW, (void*)&frow_launch<R, N, E, RW>, (const void*)&fft_rows_kernel<R, N, E, RW>},
+#define FCOL_E(R, PR, N, E, W) {PR, N, E, W, (void*)&fcol_launch<R, N, E, W>, (const void*)&fft_cols_kernel<R, N, E, W>},
+
+// Column shapes (N, E, W) and row shapes (N, E, ROWS) compiled for both precisions.
+#define COL_SHAPES(X, R, PR)                                                                              \
+  X(R, PR, 64, 8, 8) X(R, PR, 64, 8, 16) X(R, PR, 128, 8, 4) X(R, PR, 128, 8, 8) X(R, PR, 128, 8, 16)      \
+  X(R, PR, 256, 8, 4) X(R, PR, 256, 8, 8) X(R, PR, 256, 16, 8) X(R, PR, 512, 8, 2) X(R, PR, 512, 8, 4)    \
+  X(R, PR, 512, 8, 8) X(R, PR, 512, 16, 4) X(R, PR, 512, 16, 8) X(R, PR, 1024, 8, 2) X(R, PR, 1024, 8, 4) \
+  X(R, PR, 1024, 16, 4) X(R, PR, 1024, 16, 8) X(R, PR, 2048, 8, 2) X(R, PR, 2048, 8, 4)                 \
+  X(R, PR, 2048, 16, 2) X(R, PR, 2048, 16, 4) X(R, PR, 4096, 16, 2) X(R, PR, 4096, 16, 4)
+#define ROW_SHAPES(X, R, PR)                                                                                \
+  X(R, PR, 64, 8, 8) X(R, PR, 64, 8, 16) X(R, PR, 128, 8, 4) X(R, PR, 128, 8, 8) X(R, PR, 256, 8, 2)        \
+  X(R, PR, 256, 8, 4) X(R, PR, 256, 16, 4) X(R, PR, 512, 8, 1) X(R, PR, 512, 8, 2) X(R, PR, 512, 16, 2)     \
+  X(R, PR, 512, 16, 4) X(R, PR, 1024, 8, 1) X(R, PR, 1024, 16, 1) X(R, PR, 1024, 16, 2) X(R, PR, 2048, 8, 1) \
+  X(R, PR, 2048, 16, 1) X(R, PR, 4096, 16, 1)
+
+static const KEntry k_cols[] = {COL_SHAPES(COL_E, double, SGPE_C128) COL_SHAPES(COL_E, float, SGPE_C64)};
+static const KEntry k_rows[] = {ROW_SHAPES(ROW_E, double, SGPE_C128) ROW_SHAPES(ROW_E, float, SGPE_C64)};
+static const KEntry k_frows[] = {ROW_SHAPES(FROW_E, double, SGPE_C128) ROW_SHAPES(FROW_E, float, SGPE_C64)};
+static const KEntry k_fcols[] = {COL_SHAPES(FCOL_E, double, SGPE_C128) COL_SHAPES(FCOL_E, float, SGPE_C64)};
+
+template <size_t K>
+static const KEntry* find_entry(const KEntry (&tab)[K], int prec, int n, int e, int p) {
+  for (size_t i = 0; i < K; ++i)
+    if (tab[i].prec == prec && tab[i].n == n && tab[i].e == e && tab[i].p == p) return &tab[i];
+  return nullptr;
+}
+
+static const size_t kMaxLds = 160 * 1024;
+
+static size_t csize(int prec) { return prec == SGPE_C128 ? 16 : 8; }
+
+// Default shapes per length (before any tuning).
+static void default_row(int prec, int n, int* e, int* rows) {
+  (void)prec;
+  switch (n) {
+    case 64: *e = 8; *rows = 8; break;
+    case 128: *e = 8; *rows = 4; break;
+    case 256: *e = 8; *rows = 2; break;
+    case 512: *e = 8; *rows = 1; break;
+    case 1024: *e = 8; *rows = 1; break;
+    case 2048: *e = 8; *rows = 1; break;
+    default: *e = 16; *rows = 1; break;
+  }
+}
+static void default_col(int prec, int n, int* e, int* w) {
+  switch (n) {
+    case 64: *e = 8; *w = 8; break;
+    case 128: *e = 8; *w = 8; break;
+    case 256: *e = 8; *w = 4; break;
+    case 512: *e = 8; *w = 4; break;
+    case 1024: *e = 8; *w = 4; break;
+    case 2048: *e = prec == SGPE_C128 ? 16 : 8; *w = prec == SGPE_C128 ? 2 : 4; break;
+    default: *e = 16; *w = prec == SGPE_C128 ? 2 : 4; break;
+  }
+}
+
+static int set_lds_attr(const void* kernel, size_t bytes) {
+  if (bytes > 64 * 1024) HIP_TRY(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  return SGPE_OK;
+}
+
+static bool pow2_in_range(int n) { return n >= 64 && n <= 4096 && (n & (n - 1)) == 0; }
+
+// ----------------------------------------------------------------- context
+
+struct sgpe_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipStream_t cap_stream = nullptr;  // private stream used only for graph capture
+  std::map<int, void*> tw;           // key prec*65536 + n
+  double* red = nullptr;             // reduction scratch for primitives
+  size_t red_count = 0;
+  double* red_host = nullptr;        // pinned
+};
+
+static int ctx_twiddles(sgpe_ctx* ctx, int prec, int n, const void** out) {
+  const int key = prec * 65536 + n;
+  auto it = ctx->tw.find(key);
+  if (it != ctx->tw.end()) {
+    *out = it->second;
+    return SGPE_OK;
+  }
+  std::vector<double> re, im;
+  twiddles_f64(n, re, im);
+  std::vector<char> host(n * csize(prec));
+  if (prec == SGPE_C128) {
+    double* h = (double*)host.data();
+    for (int i = 0; i < n; ++i) {
+      h[2 * i] = re[i];
+      h[2 * i + 1] = im[i];
+    }
+  } else {
+    float* h = (float*)host.data();
+    for (int i = 0; i < n; ++i) {
+      h[2 * i] = (float)re[i];
+      h[2 * i + 1] = (float)im[i];
+    }
+  }
+  void* d = nullptr;
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (hipMalloc(&d, host.size()) != hipSuccess) return fail(SGPE_E_NOMEM, "twiddle table allocation failed");
+  HIP_TRY(hipMemcpy(d, host.data(), host.size(), hipMemcpyHostToDevice));
+  ctx->tw[key] = d;
+  *out = d;
+  return SGPE_OK;
+}
+
+static int ctx_reduction(sgpe_ctx* ctx, size_t count) {
+  if (ctx->red_count >= count) return SGPE_OK;
+  if (ctx->red) HIP_TRY(hipFree(ctx->red));
+  ctx->red = nullptr;
+  if (hipMalloc(&ctx->red, count * sizeof(double)) != hipSuccess) return fail(SGPE_E_NOMEM, "scratch allocation failed");
+  ctx->red_count = count;
+  return SGPE_OK;
+}
+
+// -------------------------------------------------------------------- plan
+
+struct Slot {
+  void* buf = nullptr;  // tables, f64 copies, partials of this slot
+  size_t bytes = 0;
+  bool loaded = false;
+};
+
+struct sgpe_plan {
+  sgpe_ctx* ctx = nullptr;
+  int prec = 0, ny = 0, nx = 0, nslots = 0;
+  int row_e = 8, row_rows = 1, col_e = 8, col_w = 4;
+  const KEntry* row = nullptr;
+  const KEntry* col = nullptr;
+  const void* tw_x = nullptr;  // twiddles for nx (rows)
+  const void* tw_y = nullptr;  // twiddles for ny (columns)
+  InstDev* d_inst = nullptr;
+  RunDev* d_run = nullptr;
+  std::vector<InstDev> h_inst;
+  std::vector<Slot> slots;
+  int last_dur = DUR_OUT;  // duration of the last sub-step issued (closing K pending)
+  std::map<int, hipGraphExec_t> graphs;
+  std::map<int, hipGraph_t> graph_defs;
+};
+
+static void drop_graphs(sgpe_plan* p) {
+  for (auto& g : p->graphs) (void)hipGraphExecDestroy(g.second);
+  for (auto& g : p->graph_defs) (void)hipGraphDestroy(g.second);
+  p->graphs.clear();
+  p->graph_defs.clear();
+}
+
+static int plan_select(sgpe_plan* p, int row_e, int row_rows, int col_e, int col_w) {
+  const KEntry* r = find_entry(k_rows, p->prec, p->nx, row_e, row_rows);
+  const KEntry* c = find_entry(k_cols, p->prec, p->ny, col_e, col_w);
+  if (!r || !c) return fail(SGPE_E_UNSUPPORTED, "FFT shape not compiled for this grid/precision");
+  if (p->ny % row_rows || p->nx % col_w) return fail(SGPE_E_UNSUPPORTED, "tile does not divide the grid");
+  const size_t row_lds = (size_t)row_rows * 2 * p->nx * csize(p->prec);
+  const size_t col_lds = (size_t)col_w * p->ny * csize(p->prec);
+  if (row_lds > kMaxLds || col_lds > kMaxLds) return fail(SGPE_E_UNSUPPORTED, "FFT shape exceeds LDS");
+  int rc = set_lds_attr(r->kernel, row_lds);
+  if (rc) return rc;
+  rc = set_lds_attr(c->kernel, col_lds);
+  if (rc) return rc;
+  p->row = r;
+  p->col = c;
+  p->row_e = row_e;
+  p->row_rows = row_rows;
+  p->col_e = col_e;
+  p->col_w = col_w;
+  drop_graphs(p);
+  return SGPE_OK;
+}
+
+static void launch_col(sgpe_plan* p, hipStream_t s, int mode, int dprev, int dcur) {
+  const size_t lds = std::max<size_t>((size_t)p->col_w * p->ny * csize(p->prec), 256);
+  dim3 grid(p->nx / p->col_w, 2, p->nslots), block(p->col_w * (p->ny / p->col_e));
+  ((ColFn)p->col->launch)(grid, block, lds, s, p->d_inst, p->d_run, p->tw_y, mode, dprev, dcur);
+}
+
+static void launch_row(sgpe_plan* p, hipStream_t s, int dur, int first) {
+  const size_t lds = std::max<size_t>((size_t)p->row_rows * 2 * p->nx * csize(p->prec), 256);
+  dim3 grid(p->ny / p->row_rows, 1, p->nslots), block(p->row_rows * (p->nx / p->row_e));
+  ((RowFn)p->row->launch)(grid, block, lds, s, p->d_inst, p->d_run, p->tw_x, dur, first, p->col_w);
+}
+
+static void issue_sub_step(sgpe_plan* p, hipStream_t s, int which, int first) {
+  launch_col(p, s, COL_MID, p->last_dur, which);
+  launch_row(p, s, which, first);
+  p->last_dur = which;
+}
+
+static void issue_full_step(sgpe_plan* p, hipStream_t s) {
+  issue_sub_step(p, s, DUR_OUT, 1);
+  issue_sub_step(p, s, DUR_IN, 0);
+  issue_sub_step(p, s, DUR_OUT, 0);
+}
+
+static void issue_sync(sgpe_plan* p, hipStream_t s) {
+  launch_col(p, s, COL_EXIT, p->last_dur, p->last_dur);
+  const size_t n = (size_t)2 * p->ny * p->nx;
+  const int blocks = (int)std::min<size_t>((n + 255) / 256, 1024);
+  dim3 grid(blocks, 1, p->nslots);
+  if (p->prec == SGPE_C128)
+    hipLaunchKernelGGL(normalize_kernel<double>, grid, dim3(256), 0, s, p->d_inst, p->d_run, p->col_w);
+  else
+    hipLaunchKernelGGL(normalize_kernel<float>, grid, dim3(256), 0, s, p->d_inst, p->d_run, p->col_w);
+  hipLaunchKernelGGL(set_form_kernel, dim3((p->nslots + 63) / 64), dim3(64), 0, s, p->d_run, p->nslots,
+                     (int)FORM_KSPACE);
+  p->last_dur = DUR_OUT;
+}
+
+// Host-side construction of one slot's InstDev and device buffer.
+struct Staging {
+  // Offset 0 marks an absent table, so real tables start after a header.
+  std::vector<char> bytes = std::vector<char>(256);
+  size_t add(size_t n, size_t align = 256) {
+    size_t off = (bytes.size() + align - 1) / align * align;
+    bytes.resize(off + n);
+    return off;
+  }
+};
+
+static void put_cplx(Staging& st, size_t off, int prec, int i, double re, double im) {
+  if (prec == SGPE_C128) {
+    double* d = (double*)(st.bytes.data() + off);
+    d[2 * i] = re;
+    d[2 * i + 1] = im;
+  } else {
+    float* d = (float*)(st.bytes.data() + off);
+    d[2 * i] = (float)re;
+    d[2 * i + 1] = (float)im;
+  }
+}
+
+// exp(-i e Delta): real time Delta = dt, imaginary time Delta = -i dt.
+static void phase_host(double e_dt, bool imag, double* re, double* im) {
+  if (imag) {
+    *re = std::exp(-e_dt);
+    *im = 0.0;
+  } else {
+    *re = std::cos(e_dt);
+    *im = -std::sin(e_dt);
+  }
+}
+
+static int plan_load_slot(sgpe_plan* p, int slot, const sgpe_inst_desc* d, void* psik) {
+  const int ny = p->ny, nx = p->nx, prec = p->prec;
+  const size_t cs = csize(prec);
+  if (d->ny != ny || d->nx != nx) return fail(SGPE_E_ARG, "descriptor grid does not match the plan");
+  if (!(d->t_step > 0) || !(d->atom_num > 0) || !(d->dx > 0) || !(d->dy > 0))
+    return fail(SGPE_E_ARG, "t_step, atom_num, dx, dy must be positive");
+  if (d->time_mode != SGPE_TIME_IMAG && d->time_mode != SGPE_TIME_REAL) return fail(SGPE_E_ARG, "bad time_mode");
+  if (d->coupling < 0 || d->coupling > 2) return fail(SGPE_E_ARG, "bad coupling mode");
+  if (d->coupling == SGPE_COUPLING_ARRAY && !d->omega_dense) return fail(SGPE_E_ARG, "coupling array missing");
+  for (int s = 0; s < 2; ++s) {
+    if (!d->t_dense[s] && !(d->t_x[s] && d->t_y[s])) return fail(SGPE_E_ARG, "kinetic energy missing");
+    if (!d->v_dense[s] && !(d->v_x[s] && d->v_y[s])) return fail(SGPE_E_ARG, "potential energy missing");
+  }
+  if (!psik) return fail(SGPE_E_ARG, "state pointer is NULL");
+
+  const bool imag = d->time_mode == SGPE_TIME_IMAG;
+  const double gamma = 1.0 / (2.0 + std::pow(2.0, 1.0 / 3.0));
+  const double dts[2] = {d->t_step * gamma, d->t_step * (1.0 - 2.0 * gamma)};
+
+  InstDev h;
+  std::memset(&h, 0, sizeof(h));
+  h.ny = ny;
+  h.nx = nx;
+  h.imag = imag ? 1 : 0;
+  h.coupling = d->coupling;
+  h.atom_num = d->atom_num;
+  h.g_uu = d->g_uu;
+  h.g_dd = d->g_dd;
+  h.g_ud = d->g_ud;
+  h.dv_r = d->dx * d->dy;
+  h.dv_k = (M_PI / (0.5 * nx * d->dx)) * (M_PI / (0.5 * ny * d->dy));
+  h.fft_c = d->dx * d->dy / (2.0 * M_PI);
+  h.dt_sub[0] = dts[0];
+  h.dt_sub[1] = dts[1];
+
+  Staging st;
+  size_t o_kx[2][2], o_ky[2][2], o_px[2][2], o_py[2][2], o_eth = 0;
+  size_t o_tx[2], o_ty[2], o_vx[2], o_vy[2], o_thx = 0, o_td[2], o_vd[2], o_om = 0;
+  const size_t dense = (size_t)ny * nx * sizeof(double);
+  for (int s = 0; s < 2; ++s) {
+    o_tx[s] = o_ty[s] = o_vx[s] = o_vy[s] = o_td[s] = o_vd[s] = 0;
+    if (d->t_dense[s]) {
+      o_td[s] = st.add(dense);
+      std::memcpy(st.bytes.data() + o_td[s], d->t_dense[s], dense);
+    } else {
+      o_tx[s] = st.add(nx * sizeof(double));
+      std::memcpy(st.bytes.data() + o_tx[s], d->t_x[s], nx * sizeof(double));
+      o_ty[s] = st.add(ny * sizeof(double));
+      std::memcpy(st.bytes.data() + o_ty[s], d->t_y[s], ny * sizeof(double));
+    }
+    if (d->v_dense[s]) {
+      o_vd[s] = st.add(dense);
+      std::memcpy(st.bytes.data() + o_vd[s], d->v_dense[s], dense);
+    } else {
+      o_vx[s] = st.add(nx * sizeof(double));
+      std::memcpy(st.bytes.data() + o_vx[s], d->v_x[s], nx * sizeof(double));
+      o_vy[s] = st.add(ny * sizeof(double));
+      std::memcpy(st.bytes.data() + o_vy[s], d->v_y[s], ny * sizeof(double));
+    }
+  }
+  if (d->coupling == SGPE_COUPLING_ARRAY) {
+    o_om = st.add(dense);
+    std::memcpy(st.bytes.data() + o_om, d->omega_dense, dense);
+  }
+  if (d->theta_x) {
+    o_thx = st.add(nx * sizeof(double));
+    std::memcpy(st.bytes.data() + o_thx, d->theta_x, nx * sizeof(double));
+    o_eth = st.add(nx * cs);
+  }
+  for (int dd = 0; dd < 2; ++dd)
+    for (int s = 0; s < 2; ++s) {
+      o_kx[dd][s] = o_ky[dd][s] = o_px[dd][s] = o_py[dd][s] = 0;
+      if (!d->t_dense[s]) {
+        o_kx[dd][s] = st.add(nx * cs);
+        o_ky[dd][s] = st.add(ny * cs);
+      }
+      if (!d->v_dense[s]) {
+        o_px[dd][s] = st.add(nx * cs);
+        o_py[dd][s] = st.add(ny * cs);
+      }
+    }
+  const size_t o_part = st.add((size_t)PART_SLOTS * 2 * nx * sizeof(double));
+  const size_t o_epart = st.add((size_t)EPART_SLOTS * (2 * nx + ny) * sizeof(double));
+
+  // Phase tables in float64, rounded once to the plan precision.
+  for (int dd = 0; dd < 2; ++dd) {
+    const double dt = dts[dd];
+    for (int s = 0; s < 2; ++s) {
+      double re, im;
+      if (!d->t_dense[s]) {
+        for (int i = 0; i < nx; ++i) {
+          phase_host(d->t_x[s][i] * (0.5 * dt), imag, &re, &im);
+          put_cplx(st, o_kx[dd][s], prec, i, re, im);
+        }
+        for (int i = 0; i < ny; ++i) {
+          phase_host(d->t_y[s][i] * (0.5 * dt), imag, &re, &im);
+          put_cplx(st, o_ky[dd][s], prec, i, re, im);
+        }
+      }
+      if (!d->v_dense[s]) {
+        for (int i = 0; i < nx; ++i) {
+          phase_host(d->v_x[s][i] * dt, imag, &re, &im);
+          put_cplx(st, o_px[dd][s], prec, i, re, im);
+        }
+        for (int i = 0; i < ny; ++i) {
+          phase_host(d->v_y[s][i] * dt, imag, &re, &im);
+          put_cplx(st, o_py[dd][s], prec, i, re, im);
+        }
+      }
+    }
+    // Uniform coupling: angle a = Omega Delta/4 for both durations
+    // (tensor_propagator.py:142-149 with tensor_tools.py:586).
+    const double a = d->omega * dt * 0.25;
+    if (imag) {
+      h.c_alpha[dd] = std::cosh(a);
+      h.c_beta_re[dd] = -std::sinh(a);
+      h.c_beta_im[dd] = 0.0;
+    } else {
+      h.c_alpha[dd] = std::cos(a);
+      h.c_beta_re[dd] = 0.0;
+      h.c_beta_im[dd] = -std::sin(a);
+    }
+  }
+  if (d->theta_x)
+    for (int i = 0; i < nx; ++i) put_cplx(st, o_eth, prec, i, std::cos(d->theta_x[i]), -std::sin(d->theta_x[i]));
+
+  Slot& sl = p->slots[slot];
+  if (sl.bytes < st.bytes.size()) {
+    if (sl.buf) HIP_TRY(hipFree(sl.buf));
+    sl.buf = nullptr;
+    sl.bytes = 0;
+    if (hipMalloc(&sl.buf, st.bytes.size()) != hipSuccess) return fail(SGPE_E_NOMEM, "slot table allocation failed");
+    sl.bytes = st.bytes.size();
+  }
+  char* base = (char*)sl.buf;
+  auto P = [&](size_t off) -> char* { return off ? base + off : nullptr; };
+  for (int s = 0; s < 2; ++s) {
+    h.t_dense[s] = (const double*)P(o_td[s]);
+    h.v_dense[s] = (const double*)P(o_vd[s]);
+    h.tx[s] = (const double*)P(o_tx[s]);
+    h.ty[s] = (const double*)P(o_ty[s]);
+    h.vx[s] = (const double*)P(o_vx[s]);
+    h.vy[s] = (const double*)P(o_vy[s]);
+    for (int dd = 0; dd < 2; ++dd) {
+      h.kx[dd][s] = P(o_kx[dd][s]);
+      h.ky[dd][s] = P(o_ky[dd][s]);
+      h.px[dd][s] = P(o_px[dd][s]);
+      h.py[dd][s] = P(o_py[dd][s]);
+    }
+  }
+  h.omega_dense = (const double*)P(o_om);
+  h.theta_x = (const double*)P(o_thx);
+  h.etheta = P(o_eth);
+  h.part = (double*)(base + o_part);
+  h.epart = (double*)(base + o_epart);
+  h.psi = psik;
+
+  hipStream_t s = p->ctx->stream;
+  HIP_TRY(hipMemcpyAsync(sl.buf, st.bytes.data(), st.bytes.size(), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(p->d_inst + slot, &h, sizeof(h), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(config_run_kernel, dim3(1), dim3(64), 0, s, p->d_run, slot, (double*)nullptr, 0, 1, 1,
+                     (int)FORM_KSPACE, 0.0);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s));  // staging buffer goes out of scope
+  p->h_inst[slot] = h;
+  sl.loaded = true;
+  return SGPE_OK;
+}
+
+static int plan_ready(const sgpe_plan* p) {
+  if (!p) return fail(SGPE_E_ARG, "plan is NULL");
+  for (int i = 0; i < p->nslots; ++i)
+    if (!p->slots[i].loaded) return fail(SGPE_E_ARG, "slot " + std::to_string(i) + " has not been set");
+  return SGPE_OK;
+}
+
+// ------------------------------------------------------------------- C ABI
+
+extern "C" {
+
+int sgpe_abi_version(void) { return SGPE_ABI_VERSION; }
+
+const char* sgpe_last_error(void) { return g_err.c_str(); }
+
+int sgpe_ctx_create(int device, void* stream, sgpe_ctx** out) {
+  if (!out) return fail(SGPE_E_ARG, "out is NULL");
+  *out = nullptr;
+  int n = 0;
+  HIP_TRY(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(SGPE_E_ARG, "no such device");
+  HIP_TRY(hipSetDevice(device));
+  sgpe_ctx* c = new sgpe_ctx();
+  c->device = device;
+  c->stream = (hipStream_t)stream;
+  if (hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(SGPE_E_HIP, "capture stream creation failed");
+  }
+  if (hipHostMalloc((void**)&c->red_host, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    (void)hipStreamDestroy(c->cap_stream);
+    delete c;
+    return fail(SGPE_E_HIP, "pinned buffer allocation failed");
+  }
+  *out = c;
+  return SGPE_OK;
+}
+
+int sgpe_ctx_set_stream(sgpe_ctx* ctx, void* stream) {
+  if (!ctx) return fail(SGPE_E_ARG, "ctx is NULL");
+  ctx->stream = (hipStream_t)stream;
+  return SGPE_OK;
+}
+
+int sgpe_ctx_synchronize(sgpe_ctx* ctx) {
+  if (!ctx) return fail(SGPE_E_ARG, "ctx is NULL");
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return SGPE_OK;
+}
+
+void sgpe_ctx_destroy(sgpe_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& t : ctx->tw) (void)hipFree(t.second);
+  if (ctx->red) (void)hipFree(ctx->red);
+  if (ctx->red_host) (void)hipHostFree(ctx->red_host);
+  if (ctx->cap_stream) (void)hipStreamDestroy(ctx->cap_stream);
+  delete ctx;
+}
+
+int sgpe_fft2(sgpe_ctx* ctx, int precision, int ny, int nx, int n_arrays, void* const* in, void* const* out,
+              int inverse, double dx, double dy) {
+  if (!ctx || !in || !out || n_arrays < 0) return fail(SGPE_E_ARG, "bad arguments");
+  if (precision != SGPE_C128 && precision != SGPE_C64) return fail(SGPE_E_ARG, "bad precision");
+  if (!pow2_in_range(nx) || !pow2_in_range(ny))
+    return fail(SGPE_E_UNSUPPORTED, "grid sizes must be powers of two in [64, 4096]");
+  HIP_TRY(hipSetDevice(ctx->device));
+  int re, rr, ce, cw;
+  default_row(precision, nx, &re, &rr);
+  default_col(precision, ny, &ce, &cw);
+  const KEntry* fr = find_entry(k_frows, precision, nx, re, rr);
+  const KEntry* fc = find_entry(k_fcols, precision, ny, ce, cw);
+  if (!fr || !fc) return fail(SGPE_E_UNSUPPORTED, "FFT shape not compiled");
+  const size_t rl = std::max<size_t>((size_t)rr * nx * csize(precision), 256);
+  const size_t cl = std::max<size_t>((size_t)cw * ny * csize(precision), 256);
+  int rc = set_lds_attr(fr->kernel, rl);
+  if (rc) return rc;
+  if ((rc = set_lds_attr(fc->kernel, cl))) return rc;
+  const void *twx, *twy;
+  if ((rc = ctx_twiddles(ctx, precision, nx, &twx))) return rc;
+  if ((rc = ctx_twiddles(ctx, precision, ny, &twy))) return rc;
+  const double c = dx * dy / (2.0 * M_PI);
+  dim3 rg(ny / rr), rb(rr * (nx / re)), cg(nx / cw), cb(cw * (ny / ce));
+  for (int i = 0; i < n_arrays; ++i) {
+    if (!in[i] || !out[i]) return fail(SGPE_E_ARG, "NULL array");
+    if (!inverse) {
+      ((FRowFn)fr->launch)(rg, rb, rl, ctx->stream, in[i], out[i], 0, 1, c, twx);
+      ((FColFn)fc->launch)(cg, cb, cl, ctx->stream, out[i], out[i], nx, 0, twy);
+    } else {
+      ((FColFn)fc->launch)(cg, cb, cl, ctx->stream, in[i], out[i], nx, 1, twy);
+      ((FRowFn)fr->launch)(rg, rb, rl, ctx->stream, out[i], out[i], 1, 1, 1.0 / ((double)nx * ny * c), twx);
+    }
+  }
+  HIP_TRY(hipGetLastError());
+  return SGPE_OK;
+}
+
+static int reduce_blocks(size_t n) { return (int)std::min<size_t>((n + 255) / 256, 1024); }
+
+int sgpe_norm(sgpe_ctx* ctx, int precision, int ny, int nx, void* psi0, void* psi1, double vol_elem,
+              double atom_num, void* dens0, void* dens1, double* nf_out) {
+  if (!ctx || !psi0 || !psi1 || ny <= 0 || nx <= 0) return fail(SGPE_E_ARG, "bad arguments");
+  if (precision != SGPE_C128 && precision != SGPE_C64) return fail(SGPE_E_ARG, "bad precision");
+  HIP_TRY(hipSetDevice(ctx->device));
+  const size_t n = (size_t)ny * nx;
+  const int nb = reduce_blocks(n);
+  int rc = ctx_reduction(ctx, 2 * (size_t)nb + 1);
+  if (rc) return rc;
+  double* nf_dev = ctx->red + 2 * nb;
+  if (precision == SGPE_C128) {
+    hipLaunchKernelGGL(density_sums_kernel<double>, dim3(nb), dim3(256), 0, ctx->stream, (const double2*)psi0,
+                       (const double2*)psi1, n, ctx->red);
+    hipLaunchKernelGGL(norm_apply_kernel<double>, dim3(nb), dim3(256), 0, ctx->stream, (double2*)psi0,
+                       (double2*)psi1, n, (const double*)ctx->red, nb, vol_elem, atom_num, (double*)dens0,
+                       (double*)dens1, nf_dev);
+  } else {
+    hipLaunchKernelGGL(density_sums_kernel<float>, dim3(nb), dim3(256), 0, ctx->stream, (const float2*)psi0,
+                       (const float2*)psi1, n, ctx->red);
+    hipLaunchKernelGGL(norm_apply_kernel<float>, dim3(nb), dim3(256), 0, ctx->stream, (float2*)psi0,
+                       (float2*)psi1, n, (const double*)ctx->red, nb, vol_elem, atom_num, (float*)dens0,
+                       (float*)dens1, nf_dev);
+  }
+  HIP_TRY(hipGetLastError());
+  if (nf_out) {
+    HIP_TRY(hipMemcpyAsync(ctx->red_host, nf_dev, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    *nf_out = ctx->red_host[0];
+  }
+  return SGPE_OK;
+}
+
+int sgpe_pops(sgpe_ctx* ctx, int precision, int ny, int nx, const void* psi0, const void* psi1, double vol_elem,
+              double* out2) {
+  if (!ctx || !psi0 || !psi1 || !out2 || ny <= 0 || nx <= 0) return fail(SGPE_E_ARG, "bad arguments");
+  if (precision != SGPE_C128 && precision != SGPE_C64) return fail(SGPE_E_ARG, "bad precision");
+  HIP_TRY(hipSetDevice(ctx->device));
+  const size_t n = (size_t)ny * nx;
+  const int nb = reduce_blocks(n);
+  int rc = ctx_reduction(ctx, 2 * (size_t)nb + 2);
+  if (rc) return rc;
+  double* res = ctx->red + 2 * nb;
+  if (precision == SGPE_C128)
+    hipLaunchKernelGGL(density_sums_kernel<double>, dim3(nb), dim3(256), 0, ctx->stream, (const double2*)psi0,
+                       (const double2*)psi1, n, ctx->red);
+  else
+    hipLaunchKernelGGL(density_sums_kernel<float>, dim3(nb), dim3(256), 0, ctx->stream, (const float2*)psi0,
+                       (const float2*)psi1, n, ctx->red);
+  hipLaunchKernelGGL(pops_final_kernel, dim3(1), dim3(256), 0, ctx->stream, (const double*)ctx->red, nb, vol_elem,
+                     res);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(ctx->red_host, res, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  out2[0] = ctx->red_host[0];
+  out2[1] = ctx->red_host[1];
+  return SGPE_OK;
+}
+
+int sgpe_plan_create(sgpe_ctx* ctx, int precision, int ny, int nx, int n_slots, sgpe_plan** out) {
+  if (!ctx || !out || n_slots <= 0) return fail(SGPE_E_ARG, "bad arguments");
+  *out = nullptr;
+  if (precision != SGPE_C128 && precision != SGPE_C64) return fail(SGPE_E_ARG, "bad precision");
+  if (!pow2_in_range(nx) || !pow2_in_range(ny))
+    return fail(SGPE_E_UNSUPPORTED, "grid sizes must be powers of two in [64, 4096]");
+  if (n_slots > 65535) return fail(SGPE_E_ARG, "too many slots");
+  HIP_TRY(hipSetDevice(ctx->device));
+  sgpe_plan* p = new sgpe_plan();
+  p->ctx = ctx;
+  p->prec = precision;
+  p->ny = ny;
+  p->nx = nx;
+  p->nslots = n_slots;
+  p->h_inst.resize(n_slots);
+  p->slots.resize(n_slots);
+  int re, rr, ce, cw;
+  default_row(precision, nx, &re, &rr);
+  default_col(precision, ny, &ce, &cw);
+  int rc = plan_select(p, re, rr, ce, cw);
+  if (!rc) rc = ctx_twiddles(ctx, precision, nx, &p->tw_x);
+  if (!rc) rc = ctx_twiddles(ctx, precision, ny, &p->tw_y);
+  if (!rc && hipMalloc((void**)&p->d_inst, n_slots * sizeof(InstDev)) != hipSuccess)
+    rc = fail(SGPE_E_NOMEM, "slot table allocation failed");
+  if (!rc && hipMalloc((void**)&p->d_run, n_slots * sizeof(RunDev)) != hipSuccess)
+    rc = fail(SGPE_E_NOMEM, "run state allocation failed");
+  if (!rc && hipMemset(p->d_run, 0, n_slots * sizeof(RunDev)) != hipSuccess)
+    rc = fail(SGPE_E_HIP, "run state initialization failed");
+  if (!rc && hipMemset(p->d_inst, 0, n_slots * sizeof(InstDev)) != hipSuccess)
+    rc = fail(SGPE_E_HIP, "slot table initialization failed");
+  if (rc) {
+    sgpe_plan_destroy(p);
+    return rc;
+  }
+  *out = p;
+  return SGPE_OK;
+}
+
+void sgpe_plan_destroy(sgpe_plan* p) {
+  if (!p) return;
+  (void)hipSetDevice(p->ctx->device);
+  (void)hipStreamSynchronize(p->ctx->stream);
+  drop_graphs(p);
+  for (auto& s : p->slots)
+    if (s.buf) (void)hipFree(s.buf);
+  if (p->d_inst) (void)hipFree(p->d_inst);
+  if (p->d_run) (void)hipFree(p->d_run);
+  delete p;
+}
+
+int sgpe_plan_set_slot(sgpe_plan* p, int slot, const sgpe_inst_desc* d, void* psik) {
+  if (!p || !d) return fail(SGPE_E_ARG, "bad arguments");
+  if (slot < 0 || slot >= p->nslots) return fail(SGPE_E_ARG, "slot out of range");
+  HIP_TRY(hipSetDevice(p->ctx->device));
+  return plan_load_slot(p, slot, d, psik);
+}
+
+int sgpe_plan_set_state(sgpe_plan* p, int slot, void* psik) {
+  if (!p || !psik || slot < 0 || slot >= p->nslots) return fail(SGPE_E_ARG, "bad arguments");
+  if (!p->slots[slot].loaded) return fail(SGPE_E_ARG, "slot has not been set");
+  HIP_TRY(hipSetDevice(p->ctx->device));
+  InstDev& h = p->h_inst[slot];
+  h.psi = psik;
+  HIP_TRY(hipMemcpyAsync(&p->d_inst[slot].psi, &h.psi, sizeof(void*), hipMemcpyHostToDevice, p->ctx->stream));
+  hipLaunchKernelGGL(config_run_kernel, dim3(1), dim3(64), 0, p->ctx->stream, p->d_run, slot, (double*)nullptr, 0, 1, 1,
+                     (int)FORM_KSPACE, 0.0);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(p->ctx->stream));
+  return SGPE_OK;
+}
+
+int sgpe_plan_set_run(sgpe_plan* p, int slot, double* pops, int cap, double conv_tol) {
+  if (!p || slot < 0 || slot >= p->nslots || cap < 0) return fail(SGPE_E_ARG, "bad arguments");
+  HIP_TRY(hipSetDevice(p->ctx->device));
+  hipLaunchKernelGGL(config_run_kernel, dim3(1), dim3(64), 0, p->ctx->stream, p->d_run, slot, pops, pops ? cap : 0, 1,
+                     -1, -1, conv_tol);
+  HIP_TRY(hipGetLastError());
+  return SGPE_OK;
+}
+
+int sgpe_plan_set_active(sgpe_plan* p, int slot, int active) {
+  if (!p || slot < 0 || slot >= p->nslots) return fail(SGPE_E_ARG, "bad arguments");
+  if (active && !p->slots[slot].loaded) return fail(SGPE_E_ARG, "slot has not been set");
+  HIP_TRY(hipSetDevice(p->ctx->device));
+  hipLaunchKernelGGL(set_active_kernel, dim3(1), dim3(64), 0, p->ctx->stream, p->d_run, slot, active ? 1 : 0);
+  HIP_TRY(hipGetLastError());
+  return SGPE_OK;
+}
+
+int sgpe_plan_tune(sgpe_plan* p, int row_e, int row_rows, int col_e, int col_w) {
+  if (!p) return fail(SGPE_E_ARG, "plan is NULL");
+  HIP_TRY(hipSetDevice(p->ctx->device));
+  return plan_select(p, row_e ? row_e : p->row_e, row_rows ? row_rows : p->row_rows, col_e ? col_e : p->col_e,
+                     col_w ? col_w : p->col_w);
+}
+
+int sgpe_plan_get_tuning(const sgpe_plan* p, int* out4) {
+  if (!p || !out4) return fail(SGPE_E_ARG, "bad arguments");
+  out4[0] = p->row_e;
+  out4[1] = p->row_rows;
+  out4[2] = p->col_e;
+  out4[3] = p->col_w;
+  return SGPE_OK;
+}
+
+int sgpe_single_step(sgpe_plan* p, int which) {
+  int rc = plan_ready(p);
+  if (rc) return rc;
+  if (which != SGPE_SUB_OUTER && which != SGPE_SUB_INNER) return fail(SGPE_E_ARG, "bad sub-step");
+  HIP_TRY(hipSetDevice(p->ctx->device));
+  issue_sub_step(p, p->ctx->stream, which, 0);
+  HIP_TRY(hipGetLastError());
+  return SGPE_OK;
+}
+
+int sgpe_full_steps(sgpe_plan* p, int n_steps) {
+  int rc = plan_ready(p);
+  if (rc) return rc;
+  if (n_steps < 0) return fail(SGPE_E_ARG, "n_steps < 0");
+  HIP_TRY(hipSetDevice(p->ctx->device));
+  for (int i = 0; i < n_steps; ++i) issue_full_step(p, p->ctx->stream);
+  HIP_TRY(hipGetLastError());
+  return SGPE_OK;
+}
+
+int sgpe_graph_steps(sgpe_plan* p, int k, int n_graphs) {
+  int rc = plan_ready(p);
+  if (rc) return rc;
+  if (k <= 0 || n_graphs < 0) return fail(SGPE_E_ARG, "bad step counts");
+  HIP_TRY(hipSetDevice(p->ctx->device));
+  if (n_graphs == 0) return SGPE_OK;
+  // Captured blocks assume the pending closing half-step is an outer one.
+  if (p->last_dur != DUR_OUT) issue_sync(p, p->ctx->stream);
+  auto it = p->graphs.find(k);
+  if (it == p->graphs.end()) {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    hipStream_t cs = p->ctx->cap_stream;
+    HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < k; ++i) issue_full_step(p, cs);
+    hipError_t le = hipGetLastError();
+    hipError_t ce = hipStreamEndCapture(cs, &g);
+    if (le != hipSuccess || ce != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return fail(SGPE_E_HIP, std::string("graph capture failed: ") + hipGetErrorString(le != hipSuccess ? le : ce));
+    }
+    if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      return fail(SGPE_E_HIP, "graph instantiation failed");
+    }
+    p->graph_defs[k] = g;
+    it = p->graphs.emplace(k, ge).first;
+  }
+  for (int i = 0; i < n_graphs; ++i) HIP_TRY(hipGraphLaunch(it->second, p->ctx->stream));
+  p->last_dur = DUR_OUT;
+  return SGPE_OK;
+}
+
+int sgpe_plan_sync(sgpe_plan* p) {
+  int rc = plan_ready(p);
+  if (rc) return rc;
+  HIP_TRY(hipSetDevice(p->ctx->device));
+  issue_sync(p, p->ctx->stream);
+  HIP_TRY(hipGetLastError());
+  return SGPE_OK;
+}
+
+int sgpe_energy(sgpe_plan* p, double* out) {
+  (void)out;
+  int rc = plan_ready(p);
+  if (rc) return rc;
+  return fail(SGPE_E_UNSUPPORTED, "energy evaluation is not available in this build");
+}
+
+}  // extern "C"

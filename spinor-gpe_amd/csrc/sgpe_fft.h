@@ -1,0 +1,224 @@
+// In-register mixed-radix FFT for power-of-two lengths 64..4096.
+//
+// Schedule: Stockham auto-sort, natural order in and out. A transform of
+// length N is done by T = N/E threads; thread t holds a[j] = X[t + j*T]
+// (j < E). Each stage of radix R <= E and span L turns the virtual thread
+// v = t + u*T (u < E/R) into one radix-R butterfly on X[v + r*N/R], twiddled
+// by w_{L*R}^{r*(v mod L)}, and scatters the outputs to Y[(v - k)*R + k + q*L].
+// The first stage needs no twiddles, the last stage's outputs already sit at
+// t + j*T, so a transform touches LDS only between stages and its global
+// loads/stores are unit-stride across threads (coalesced).
+//
+// Unnormalized: forward uses exp(-2 pi i/N), inverse exp(+2 pi i/N), no 1/N.
+// Twiddles come from a host-built table tw[m] = exp(-2 pi i m/N) (rounded
+// from float64 values computed with exact octant reduction).
+#pragma once
+
+#include "sgpe_common.h"
+
+namespace sgpe {
+
+// Multiply by -i (forward) or +i (inverse).
+template <bool INV, typename C> __device__ __forceinline__ C rot90(C a) {
+  return INV ? C{-a.y, a.x} : C{a.y, -a.x};
+}
+
+template <bool INV, typename C> __device__ __forceinline__ void dft2(C& a, C& b) {
+  C s = cadd(a, b), d = csub(a, b);
+  a = s;
+  b = d;
+}
+
+template <bool INV, typename C> __device__ __forceinline__ void dft4(C& b0, C& b1, C& b2, C& b3) {
+  C t0 = cadd(b0, b2), t1 = csub(b0, b2);
+  C t2 = cadd(b1, b3), t3 = rot90<INV>(csub(b1, b3));
+  b0 = cadd(t0, t2);
+  b2 = csub(t0, t2);
+  b1 = cadd(t1, t3);
+  b3 = csub(t1, t3);
+}
+
+// w8^1 * a and w8^3 * a with w8 = exp(-/+ i pi/4).
+template <bool INV, typename C, typename R> __device__ __forceinline__ C w8_1(C a, R h) {
+  return INV ? C{(a.x - a.y) * h, (a.x + a.y) * h} : C{(a.x + a.y) * h, (a.y - a.x) * h};
+}
+template <bool INV, typename C, typename R> __device__ __forceinline__ C w8_3(C a, R h) {
+  return INV ? C{-(a.x + a.y) * h, (a.x - a.y) * h} : C{(a.y - a.x) * h, -(a.x + a.y) * h};
+}
+
+template <bool INV, typename C> __device__ __forceinline__ void dft8(C* b) {
+  typedef decltype(b[0].x) R;
+  const R h = (R)0.70710678118654752440;  // 1/sqrt(2)
+  C e0 = b[0], e1 = b[2], e2 = b[4], e3 = b[6];
+  C o0 = b[1], o1 = b[3], o2 = b[5], o3 = b[7];
+  dft4<INV>(e0, e1, e2, e3);
+  dft4<INV>(o0, o1, o2, o3);
+  o1 = w8_1<INV>(o1, h);
+  o2 = rot90<INV>(o2);
+  o3 = w8_3<INV>(o3, h);
+  b[0] = cadd(e0, o0);
+  b[4] = csub(e0, o0);
+  b[1] = cadd(e1, o1);
+  b[5] = csub(e1, o1);
+  b[2] = cadd(e2, o2);
+  b[6] = csub(e2, o2);
+  b[3] = cadd(e3, o3);
+  b[7] = csub(e3, o3);
+}
+
+template <bool INV, typename C> __device__ __forceinline__ void dft16(C* b) {
+  typedef decltype(b[0].x) R;
+  const R h = (R)0.70710678118654752440;   // cos(pi/4)
+  const R c1 = (R)0.92387953251128675613;  // cos(pi/8)
+  const R s1 = (R)0.38268343236508977173;  // sin(pi/8)
+  const R sg = INV ? (R)1 : (R)-1;         // sign of the twiddle's imaginary part
+  C e[8], o[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    e[i] = b[2 * i];
+    o[i] = b[2 * i + 1];
+  }
+  dft8<INV>(e);
+  dft8<INV>(o);
+  // o[q] *= w16^q
+  o[1] = cmul(o[1], C{c1, sg * s1});
+  o[2] = w8_1<INV>(o[2], h);
+  o[3] = cmul(o[3], C{s1, sg * c1});
+  o[4] = rot90<INV>(o[4]);
+  o[5] = cmul(o[5], C{-s1, sg * c1});
+  o[6] = w8_3<INV>(o[6], h);
+  o[7] = cmul(o[7], C{-c1, sg * s1});
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    b[q] = cadd(e[q], o[q]);
+    b[q + 8] = csub(e[q], o[q]);
+  }
+}
+
+template <int R, bool INV, typename C> __device__ __forceinline__ void dft(C* b) {
+  if constexpr (R == 2) {
+    dft2<INV>(b[0], b[1]);
+  } else if constexpr (R == 4) {
+    dft4<INV>(b[0], b[1], b[2], b[3]);
+  } else if constexpr (R == 8) {
+    dft8<INV>(b);
+  } else {
+    static_assert(R == 16, "radix must be 2, 4, 8 or 16");
+    dft16<INV>(b);
+  }
+}
+
+// LDS view of one transform: element i lives at base[i*STRIDE + (i/PADB)*STRIDE*PADW].
+// STRIDE interleaves several transforms (columns); PADB/PADW insert padding
+// to break power-of-two bank strides (PADW = 0: none).
+template <int STRIDE, int PADB, int PADW> struct LdsView {
+  static __device__ __forceinline__ int at(int i) { return (i + (PADW ? (i / PADB) * PADW : 0)) * STRIDE; }
+  static constexpr int span(int n) { return (n + (PADW ? (n / PADB) * PADW : 0)) * STRIDE; }
+};
+
+// One Stockham stage. Non-final stages write their outputs to LDS; the final
+// stage leaves output Y[t + j*T] in a[j].
+template <int N, int E, int R, int L, bool INV, bool LAST, class V, typename C>
+__device__ __forceinline__ void fft_stage(C (&a)[E], int t, const C* __restrict__ tw, C* lds) {
+  constexpr int T = N / E;
+  constexpr int U = E / R;
+  C out[LAST ? E : 1];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int v = t + u * T;
+    const int k = v & (L - 1);
+    C b[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) b[r] = a[u + r * U];
+    if constexpr (L > 1) {
+      constexpr int step = N / (L * R);
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        C w = tw[r * k * step];
+        if (INV) w.y = -w.y;
+        b[r] = cmul(b[r], w);
+      }
+    }
+    dft<R, INV>(b);
+    if constexpr (LAST) {
+#pragma unroll
+      for (int q = 0; q < R; ++q) out[u + q * U] = b[q];
+    } else {
+      const int base = (v - k) * R + k;
+#pragma unroll
+      for (int q = 0; q < R; ++q) lds[V::at(base + q * L)] = b[q];
+    }
+  }
+  if constexpr (LAST) {
+#pragma unroll
+    for (int j = 0; j < E; ++j) a[j] = out[j];
+  }
+}
+
+// Full transform: stages of radix E while N/L >= E, then the remainder radix.
+// Every thread of the block must call this (it contains barriers).
+template <int N, int E, bool INV, class V, int L = 1, typename C>
+__device__ __forceinline__ void fft_run(C (&a)[E], int t, const C* __restrict__ tw, C* lds) {
+  constexpr int R = (N / L >= E) ? E : N / L;
+  constexpr bool LAST = (L * R == N);
+  constexpr int T = N / E;
+  fft_stage<N, E, R, L, INV, LAST, V>(a, t, tw, lds);
+  if constexpr (!LAST) {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < E; ++j) a[j] = lds[V::at(t + j * T)];
+    __syncthreads();
+    fft_run<N, E, INV, V, L * R>(a, t, tw, lds);
+  }
+}
+
+// Two independent transforms (e.g. both spinor components of one row) that
+// share barriers: lds0/lds1 are disjoint views.
+template <int N, int E, bool INV, class V, int L = 1, typename C>
+__device__ __forceinline__ void fft_run2(C (&a)[E], C (&b)[E], int t, const C* __restrict__ tw, C* lds0,
+                                         C* lds1) {
+  constexpr int R = (N / L >= E) ? E : N / L;
+  constexpr bool LAST = (L * R == N);
+  constexpr int T = N / E;
+  fft_stage<N, E, R, L, INV, LAST, V>(a, t, tw, lds0);
+  fft_stage<N, E, R, L, INV, LAST, V>(b, t, tw, lds1);
+  if constexpr (!LAST) {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      a[j] = lds0[V::at(t + j * T)];
+      b[j] = lds1[V::at(t + j * T)];
+    }
+    __syncthreads();
+    fft_run2<N, E, INV, V, L * R>(a, b, t, tw, lds0, lds1);
+  }
+}
+
+// Deterministic block-wide sum of NV doubles per thread (fixed tree order).
+// `scratch` needs NV * (blockDim.x / 64) doubles. All threads get the result.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* scratch) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nwave = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v[i] += __shfl_xor(v[i], off, 64);
+  }
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) scratch[i * nwave + wave] = v[i];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double s = 0.0;
+    for (int w = 0; w < nwave; ++w) s += scratch[i * nwave + w];
+    v[i] = s;
+  }
+  __syncthreads();
+}
+
+}  // namespace sgpe

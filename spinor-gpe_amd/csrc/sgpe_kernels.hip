@@ -1,0 +1,455 @@
+// Split-step kernels. One split step (the reference's full_step,
+// spinor_gpe/pspinor/tensor_propagator.py:214-222) is three sub-steps; each
+// sub-step here is two sweeps over the state:
+//
+//   column pass  (per column jx, per component; y is the strided axis)
+//       DFT_y -> K_prev (end of the previous sub-step) -> [S: sum |.|^2]
+//             -> K_cur (start of this sub-step) -> [NRM: sum |.|^2] -> IDFT_y
+//   row pass     (per row y, both components together)
+//       IDFT_x -> normalize (from NRM) -> U C P C U -> DFT_x * dx dy/(2 pi)
+//
+// which is tensor_propagator.py:242-271 with the fftshift/ifftshift pairs and
+// the FFT scale factors folded away: for even grid sizes the shifts only
+// multiply real space by (-1)^(x+y), which commutes with every real-space
+// factor, and the scale is absorbed by the renormalization that follows each
+// inverse transform. The k-space renormalization at the end of a sub-step
+// (tensor_propagator.py:271) is deferred: the next real-space renormalization
+// absorbs it, the per-step populations are ratios of the S sums, and the
+// explicit normalize kernel applies it when the state is handed back.
+#include "sgpe_fft.h"
+
+namespace sgpe {
+
+// ---------------------------------------------------------------- helpers
+
+template <typename R> __device__ __forceinline__ void sc(R x, R* s, R* c);
+template <> __device__ __forceinline__ void sc<double>(double x, double* s, double* c) { sincos(x, s, c); }
+template <> __device__ __forceinline__ void sc<float>(float x, float* s, float* c) { sincosf(x, s, c); }
+template <typename R> __device__ __forceinline__ R ex(R x);
+template <> __device__ __forceinline__ double ex<double>(double x) { return exp(x); }
+template <> __device__ __forceinline__ float ex<float>(float x) { return expf(x); }
+
+// exp(-i E Delta) for Delta = dt (real time) or -i dt (imaginary time).
+template <typename R>
+__device__ __forceinline__ typename Cx<R>::T phase_op(R e_dt, bool imag) {
+  typedef typename Cx<R>::T C;
+  if (imag) return C{ex<R>(-e_dt), (R)0};
+  R s, c;
+  sc<R>(-e_dt, &s, &c);
+  return C{c, s};
+}
+
+template <typename R> __device__ __forceinline__ double norm2d(typename Cx<R>::T a) {
+  double x = (double)a.x, y = (double)a.y;
+  return x * x + y * y;
+}
+
+// ------------------------------------------------------------ column pass
+
+template <typename R, int N, int E, int W>
+__global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __restrict__ insts,
+                                                             RunDev* __restrict__ runs,
+                                                             const typename Cx<R>::T* __restrict__ tw,
+                                                             int mode, int d_prev, int d_cur) {
+  typedef typename Cx<R>::T C;
+  typedef LdsView<W, 1 << 30, 0> V;
+  constexpr int T = N / E;
+  extern __shared__ double smem[];
+  const int inst = blockIdx.z;
+  if (!runs[inst].active) return;
+  const bool kspace = runs[inst].form == FORM_KSPACE;
+  const InstDev& P = insts[inst];
+  const int nx = P.nx;
+  const int comp = blockIdx.y;
+  const int tile = blockIdx.x;
+  const int c = threadIdx.x % W;
+  const int t = threadIdx.x / W;
+  const int jx = tile * W + c;
+  C* lds = reinterpret_cast<C*>(smem) + c;
+  C* col = reinterpret_cast<C*>(P.psi) + (size_t)comp * N * nx + jx;
+
+  C a[E];
+#pragma unroll
+  for (int j = 0; j < E; ++j) a[j] = col[(size_t)(t + j * T) * nx];
+
+  if (!kspace) fft_run<N, E, false, V>(a, t, tw, lds);
+
+  const bool imag = P.imag != 0;
+  const double* td = P.t_dense[comp];
+  double s_acc = 0.0, n_acc = 0.0;
+  C kxp = C{(R)1, (R)0}, kxc = C{(R)1, (R)0};
+  const C* kyp = nullptr;
+  const C* kyc = nullptr;
+  if (!td) {
+    kxp = reinterpret_cast<const C*>(P.kx[d_prev][comp])[jx];
+    kyp = reinterpret_cast<const C*>(P.ky[d_prev][comp]);
+    kxc = reinterpret_cast<const C*>(P.kx[d_cur][comp])[jx];
+    kyc = reinterpret_cast<const C*>(P.ky[d_cur][comp]);
+  }
+  const R hp = (R)(0.5 * P.dt_sub[d_prev]);
+  const R hc = (R)(0.5 * P.dt_sub[d_cur]);
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const int jy = t + j * T;
+    R tval = td ? (R)td[(size_t)jy * nx + jx] : (R)0;
+    if (!kspace) {
+      C k = td ? phase_op<R>(tval * hp, imag) : cmul(kxp, kyp[jy]);
+      a[j] = cmul(a[j], k);
+    }
+    s_acc += norm2d<R>(a[j]);
+    if (mode == COL_MID) {
+      C k = td ? phase_op<R>(tval * hc, imag) : cmul(kxc, kyc[jy]);
+      a[j] = cmul(a[j], k);
+      n_acc += norm2d<R>(a[j]);
+    }
+  }
+  if (mode == COL_MID) fft_run<N, E, true, V>(a, t, tw, lds);
+#pragma unroll
+  for (int j = 0; j < E; ++j) col[(size_t)(t + j * T) * nx] = a[j];
+
+  double v[2] = {s_acc, n_acc};
+  block_sum<2>(v, smem);
+  if (threadIdx.x == 0) {
+    const int nt = P.nx / W;
+    P.part[PART_S * 2 * nt + comp * nt + tile] = v[0];
+    P.part[PART_NRM * 2 * nt + comp * nt + tile] = v[1];
+  }
+}
+
+// Sum of one partial slot over all column tiles of both components
+// (deterministic: same order in every block). All threads get the result.
+__device__ __forceinline__ void part_sums(const double* part, int nt2, int slot, int split, double* out2,
+                                          double* scratch) {
+  double v[2] = {0.0, 0.0};
+  const double* p = part + (size_t)slot * nt2;
+  for (int i = threadIdx.x; i < nt2; i += blockDim.x) v[(split && i >= nt2 / 2) ? 1 : 0] += p[i];
+  block_sum<2>(v, scratch);
+  out2[0] = v[0];
+  out2[1] = v[1];
+}
+
+// ---------------------------------------------------------------- row pass
+
+template <typename R, int N, int E, int ROWS>
+__global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* __restrict__ insts,
+                                                                RunDev* __restrict__ runs,
+                                                                const typename Cx<R>::T* __restrict__ tw,
+                                                                int dur, int first_sub, int col_w) {
+  typedef typename Cx<R>::T C;
+  typedef LdsView<1, 1 << 30, 0> V;
+  constexpr int T = N / E;
+  extern __shared__ double smem[];
+  const int inst = blockIdx.z;
+  RunDev& run = runs[inst];
+  if (!run.active) return;
+  const InstDev& P = insts[inst];
+  const int ny = P.ny;
+  const int nt2 = 2 * (N / col_w);
+
+  // Real-space normalization (tensor_tools.py:301-305) from the column
+  // pass's k-space sums by Parseval: sum_r |IDFT2 v|^2 = Nx Ny sum_k |v|^2.
+  double nrm[2];
+  part_sums(P.part, nt2, PART_NRM, 0, nrm, smem);
+  if (first_sub && blockIdx.x == 0) {
+    double s[2];
+    part_sums(P.part, nt2, PART_S, 1, s, smem);
+    if (threadIdx.x == 0) {
+      const int step = run.step;
+      if (step > 0 && run.pops && step - 1 < run.pops_cap) {
+        const double tot = s[0] + s[1];
+        run.pops[2 * (step - 1) + 0] = P.atom_num * s[0] / tot;
+        run.pops[2 * (step - 1) + 1] = P.atom_num * s[1] / tot;
+      }
+      run.step = step + 1;
+      run.form = FORM_HYBRID;
+    }
+  }
+  const double lam2 = P.atom_num / (P.dv_r * (double)N * (double)ny * nrm[0]);
+
+  const int row = threadIdx.x / T;
+  const int t = threadIdx.x % T;
+  const int y = blockIdx.x * ROWS + row;
+  C* r0 = reinterpret_cast<C*>(P.psi) + (size_t)y * N;
+  C* r1 = r0 + (size_t)ny * N;
+  C* lds0 = reinterpret_cast<C*>(smem) + (size_t)row * 2 * N;
+  C* lds1 = lds0 + N;
+
+  C a0[E], a1[E];
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    a0[j] = r0[t + j * T];
+    a1[j] = r1[t + j * T];
+  }
+  fft_run2<N, E, true, V>(a0, a1, t, tw, lds0, lds1);
+
+  const bool imag = P.imag != 0;
+  const R lam = (R)sqrt(lam2);
+  const R lam2r = (R)lam2;
+  const R cf = (R)P.fft_c;
+  const R guu = (R)P.g_uu, gdd = (R)P.g_dd, gud = (R)P.g_ud;
+  const R hdt = (R)(0.5 * P.dt_sub[dur]);
+  const R dts = (R)P.dt_sub[dur];
+  const double* vd0 = P.v_dense[0];
+  const double* vd1 = P.v_dense[1];
+  C py0 = C{(R)1, (R)0}, py1 = C{(R)1, (R)0};
+  const C* px0 = nullptr;
+  const C* px1 = nullptr;
+  if (!vd0) {
+    py0 = reinterpret_cast<const C*>(P.py[dur][0])[y];
+    px0 = reinterpret_cast<const C*>(P.px[dur][0]);
+  }
+  if (!vd1) {
+    py1 = reinterpret_cast<const C*>(P.py[dur][1])[y];
+    px1 = reinterpret_cast<const C*>(P.px[dur][1]);
+  }
+  const int cpl = P.coupling;
+  const R ca = (R)P.c_alpha[dur];
+  const C cb = C{(R)P.c_beta_re[dur], (R)P.c_beta_im[dur]};
+  const C* eth = reinterpret_cast<const C*>(P.etheta);
+  const double* omd = P.omega_dense;
+
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const int x = t + j * T;
+    C p0 = a0[j], p1 = a1[j];
+    // Interaction half step U_s = exp(-i (g n) Delta/2) from the normalized
+    // densities (tensor_propagator.py:247-250); U is reused after C P C (:267).
+    const R n0 = lam2r * (p0.x * p0.x + p0.y * p0.y);
+    const R n1 = lam2r * (p1.x * p1.x + p1.y * p1.y);
+    const C u0 = phase_op<R>((guu * n0 + gud * n1) * hdt, imag);
+    const C u1 = phase_op<R>((gdd * n1 + gud * n0) * hdt, imag);
+    p0 = cscale(cmul(p0, u0), lam);
+    p1 = cscale(cmul(p1, u1), lam);
+    R alpha = ca;
+    C b01 = cb, b10 = cb;
+    if (cpl) {
+      if (cpl == 2) {
+        const R ang = (R)omd[(size_t)y * N + x] * dts * (R)0.25;
+        if (imag) {
+          const R e = ex<R>(ang), ei = (R)1 / e;
+          alpha = (R)0.5 * (e + ei);
+          b01 = C{(R)-0.5 * (e - ei), (R)0};
+        } else {
+          R s, c;
+          sc<R>(ang, &s, &c);
+          alpha = c;
+          b01 = C{(R)0, -s};
+        }
+        b10 = b01;
+      }
+      if (eth) {
+        const C e = eth[x];
+        b10 = cmul(b01, cconj(e));
+        b01 = cmul(b01, e);
+      }
+      const C q0 = cadd(cscale(p0, alpha), cmul(b01, p1));
+      const C q1 = cadd(cmul(b10, p0), cscale(p1, alpha));
+      p0 = q0;
+      p1 = q1;
+    }
+    // Potential step P_s = exp(-i V_s Delta) (tensor_propagator.py:256).
+    const C q0 = vd0 ? phase_op<R>((R)vd0[(size_t)y * N + x] * dts, imag) : cmul(px0[x], py0);
+    const C q1 = vd1 ? phase_op<R>((R)vd1[(size_t)y * N + x] * dts, imag) : cmul(px1[x], py1);
+    p0 = cmul(p0, q0);
+    p1 = cmul(p1, q1);
+    if (cpl) {
+      const C s0 = cadd(cscale(p0, alpha), cmul(b01, p1));
+      const C s1 = cadd(cmul(b10, p0), cscale(p1, alpha));
+      p0 = s0;
+      p1 = s1;
+    }
+    a0[j] = cscale(cmul(p0, u0), cf);
+    a1[j] = cscale(cmul(p1, u1), cf);
+  }
+  fft_run2<N, E, false, V>(a0, a1, t, tw, lds0, lds1);
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    r0[t + j * T] = a0[j];
+    r1[t + j * T] = a1[j];
+  }
+}
+
+// --------------------------------------------------------------- normalize
+// k-space renormalization to atom_num (tensor_propagator.py:271) from the
+// exit column pass's S sums, plus the populations of the last step
+// (tensor_propagator.py:194). Grid: (chunks, 1, n_inst).
+template <typename R>
+__global__ __launch_bounds__(256) void normalize_kernel(const InstDev* __restrict__ insts,
+                                                        RunDev* __restrict__ runs, int col_w) {
+  typedef typename Cx<R>::T C;
+  __shared__ double scratch[16];
+  const int inst = blockIdx.z;
+  RunDev& run = runs[inst];
+  if (!run.active) return;
+  const InstDev& P = insts[inst];
+  const int nt2 = 2 * (P.nx / col_w);
+  double s[2];
+  part_sums(P.part, nt2, PART_S, 1, s, scratch);
+  const double tot = s[0] + s[1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const int step = run.step;
+    if (step > 0 && run.pops && step - 1 < run.pops_cap) {
+      run.pops[2 * (step - 1) + 0] = P.atom_num * s[0] / tot;
+      run.pops[2 * (step - 1) + 1] = P.atom_num * s[1] / tot;
+    }
+  }
+  const R kappa = (R)sqrt(P.atom_num / (P.dv_k * tot));
+  C* psi = reinterpret_cast<C*>(P.psi);
+  const size_t n = (size_t)2 * P.ny * P.nx;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    psi[i] = cscale(psi[i], kappa);
+}
+
+// Marks the state as normalized k-space. Separate launch so that no block of
+// the normalize kernel can observe a half-updated flag.
+__global__ void set_form_kernel(RunDev* __restrict__ runs, int n, int form) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && runs[i].active) runs[i].form = form;
+}
+
+__global__ void set_active_kernel(RunDev* __restrict__ runs, int slot, int active) {
+  if (threadIdx.x == 0) runs[slot].active = active;
+}
+
+// Run-state (re)configuration for one slot; launched instead of copying so
+// that it is ordered on the stream and graph-safe.
+__global__ void config_run_kernel(RunDev* __restrict__ runs, int slot, double* pops, int cap, int reset_step,
+                                  int active, int form, double conv_tol) {
+  if (threadIdx.x != 0) return;
+  RunDev& r = runs[slot];
+  r.pops = pops;
+  r.pops_cap = cap;
+  if (reset_step) {
+    r.step = 0;
+    r.converged = 0;
+    r.n_checks = 0;
+    r.e_prev = 0.0;
+    r.steps_prev = 0;
+  }
+  if (active >= 0) r.active = active;
+  if (form >= 0) r.form = form;
+  r.conv_tol = conv_tol;
+}
+
+// ------------------------------------------------- standalone primitives
+// Device forms of tensor_tools.fft_2d / ifft_2d (tensor_tools.py:201-258):
+//   fft_2d : fftshift(FFTN(psi)) * c  = DFT_y DFT_x [(-1)^(x+y) c psi]
+//   ifft_2d: IFFTN(ifftshift(psik))/c = (-1)^(x+y) / (Nx Ny c) IDFT_x IDFT_y psik
+// Rows: optional pre- (forward) or post- (inverse) factor (-1)^(x+y) * scale.
+template <typename R, int N, int E, int ROWS>
+__global__ __launch_bounds__(ROWS*(N / E)) void fft_rows_kernel(const typename Cx<R>::T* __restrict__ in,
+                                                                typename Cx<R>::T* __restrict__ out, int inv,
+                                                                int shift, double scale,
+                                                                const typename Cx<R>::T* __restrict__ tw) {
+  typedef typename Cx<R>::T C;
+  typedef LdsView<1, 1 << 30, 0> V;
+  constexpr int T = N / E;
+  extern __shared__ double smem[];
+  const int row = threadIdx.x / T;
+  const int t = threadIdx.x % T;
+  const int y = blockIdx.x * ROWS + row;
+  const C* src = in + (size_t)y * N;
+  C* dst = out + (size_t)y * N;
+  C* lds = reinterpret_cast<C*>(smem) + (size_t)row * N;
+  C a[E];
+#pragma unroll
+  for (int j = 0; j < E; ++j) a[j] = src[t + j * T];
+  // (-1)^(x+y) with x = t + j*T: T is even, so the sign depends on t and y only.
+  const R f = (R)(shift && ((t + y) & 1) ? -scale : scale);
+  if (!inv) {
+#pragma unroll
+    for (int j = 0; j < E; ++j) a[j] = cscale(a[j], f);
+    fft_run<N, E, false, V>(a, t, tw, lds);
+  } else {
+    fft_run<N, E, true, V>(a, t, tw, lds);
+#pragma unroll
+    for (int j = 0; j < E; ++j) a[j] = cscale(a[j], f);
+  }
+#pragma unroll
+  for (int j = 0; j < E; ++j) dst[t + j * T] = a[j];
+}
+
+template <typename R, int N, int E, int W>
+__global__ __launch_bounds__(W*(N / E)) void fft_cols_kernel(const typename Cx<R>::T* __restrict__ in,
+                                                             typename Cx<R>::T* __restrict__ out, int nx,
+                                                             int inv, const typename Cx<R>::T* __restrict__ tw) {
+  typedef typename Cx<R>::T C;
+  typedef LdsView<W, 1 << 30, 0> V;
+  constexpr int T = N / E;
+  extern __shared__ double smem[];
+  const int c = threadIdx.x % W;
+  const int t = threadIdx.x / W;
+  const int jx = blockIdx.x * W + c;
+  C* lds = reinterpret_cast<C*>(smem) + c;
+  C a[E];
+#pragma unroll
+  for (int j = 0; j < E; ++j) a[j] = in[(size_t)(t + j * T) * nx + jx];
+  if (inv)
+    fft_run<N, E, true, V>(a, t, tw, lds);
+  else
+    fft_run<N, E, false, V>(a, t, tw, lds);
+#pragma unroll
+  for (int j = 0; j < E; ++j) out[(size_t)(t + j * T) * nx + jx] = a[j];
+}
+
+// Per-block sums of |psi0|^2 and |psi1|^2 (float64), written to part[2*block + s].
+template <typename R>
+__global__ __launch_bounds__(256) void density_sums_kernel(const typename Cx<R>::T* __restrict__ p0,
+                                                           const typename Cx<R>::T* __restrict__ p1,
+                                                           size_t n, double* __restrict__ part) {
+  __shared__ double scratch[16];
+  double v[2] = {0.0, 0.0};
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    v[0] += norm2d<R>(p0[i]);
+    v[1] += norm2d<R>(p1[i]);
+  }
+  block_sum<2>(v, scratch);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x + 0] = v[0];
+    part[2 * blockIdx.x + 1] = v[1];
+  }
+}
+
+// norm() (tensor_tools.py:261-311): nf = sum(dens0 + dens1) * vol / atom_num;
+// psi /= sqrt(nf); dens = |psi_before|^2 / nf. Every block reduces the same
+// partials in the same order, so all blocks use the same nf.
+template <typename R>
+__global__ __launch_bounds__(256) void norm_apply_kernel(typename Cx<R>::T* __restrict__ p0,
+                                                         typename Cx<R>::T* __restrict__ p1, size_t n,
+                                                         const double* __restrict__ part, int nblk, double vol,
+                                                         double atom_num, R* __restrict__ d0,
+                                                         R* __restrict__ d1, double* __restrict__ nf_out) {
+  __shared__ double scratch[16];
+  double v[2] = {0.0, 0.0};
+  for (int i = threadIdx.x; i < nblk; i += blockDim.x) v[0] += part[2 * i] + part[2 * i + 1];
+  block_sum<2>(v, scratch);
+  const double nf = v[0] * vol / atom_num;
+  if (nf_out && blockIdx.x == 0 && threadIdx.x == 0) *nf_out = nf;
+  const R inv_sqrt = (R)(1.0 / sqrt(nf));
+  const R inv_nf = (R)(1.0 / nf);
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const typename Cx<R>::T a = p0[i], b = p1[i];
+    if (d0) d0[i] = (R)((a.x * a.x + a.y * a.y) * inv_nf);
+    if (d1) d1[i] = (R)((b.x * b.x + b.y * b.y) * inv_nf);
+    p0[i] = cscale(a, inv_sqrt);
+    p1[i] = cscale(b, inv_sqrt);
+  }
+}
+
+// Final reduction of density_sums partials into out2[s] = vol * sum |psi_s|^2.
+__global__ __launch_bounds__(256) void pops_final_kernel(const double* __restrict__ part, int nblk, double vol,
+                                                         double* __restrict__ out2) {
+  __shared__ double scratch[16];
+  double v[2] = {0.0, 0.0};
+  for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
+    v[0] += part[2 * i];
+    v[1] += part[2 * i + 1];
+  }
+  block_sum<2>(v, scratch);
+  if (threadIdx.x == 0) {
+    out2[0] = v[0] * vol;
+    out2[1] = v[1] * vol;
+  }
+}
+
+}  // namespace sgpe
