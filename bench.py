@@ -1,0 +1,197 @@
+"""Benchmark: split-steps/s of the 512^2 real-time Raman-coupled propagation.
+
+Workload (one instance on one GPU): 512 x 512 grid over +/-24 oscillator
+lengths, isotropic 50 Hz trap, 1e4 atoms, g = (1, 1, 0.995) x g_scale, Raman
+coupling Omega = 2 E_R, detuning 0.5 E_R, kL = 3 along x (lab frame),
+dt = 1e-3. The initial state is the imaginary-time ground state of the
+256^2 / +/-16 configuration (4000 steps, dt = 0.02, Thomas-Fermi start with
+seeded +/-0.1 rad phase noise) resampled to the 512^2 grid, times seeded
+Gaussian phase noise (sigma 0.05). A step is one full split step (three
+sub-steps) of the whole state; inputs are resident in HBM before timing.
+
+Prints one JSON line. ``value`` is the complex128 rate (the reference's
+precision); the complex64 rate is reported under ``"c64"``. ``roofline``
+prices the dominant kernel's algorithmic bytes (read + write of the state,
+2 * 2 * Nx * Ny * 16 B) against the 8 TB/s HBM peak, with per-kernel average
+durations from HIP events on the launch stream. ``cpu_baseline`` times the
+CPU oracle (an op-for-op torch restatement of the reference's path) on a
+bounded sample of the same workload.
+"""
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (os.path.join(ROOT, 'spinor-gpe_amd'), ROOT):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import torch  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+TITAN_V_512 = 149.1  # split-steps/s, bench_data/TitanV_cuda_step.npz (512^2, complex128)
+W50 = 2 * np.pi * 50
+WAVEL = 790.1e-9
+METRIC = ("split-steps/s at 512² (1 GPU); sweep solves/s + p50/p99 latency at "
+          "ΔE≤1e-6")
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def ground_state_256(device):
+    """Imaginary-time ground state of the 256^2 / +/-16 configuration."""
+    from spinor_gpe.pspinor import pspinor as spin
+    ps = spin.PSpinor('c0', omeg={'x': W50, 'y': W50, 'z': 40 * W50},
+                      g_sc={'uu': 1.0, 'dd': 1.0, 'ud': 0.995}, mesh_points=(256, 256),
+                      r_sizes=(16, 16), atom_num=1e4, pop_frac=(0.5, 0.5))
+    rng = np.random.default_rng(0)
+    ph = rng.uniform(-0.1, 0.1, (2, 256, 256))
+    ps.psi = [p * np.exp(1j * f) for p, f in zip(ps.psi, ph)]
+    c = ps.space['dv_r'] / (2 * np.pi)
+    ps.psik = [np.fft.fftshift(np.fft.fftn(p) * c) for p in ps.psi]
+    res, _ = ps.imaginary(0.02, 4000, device, progress=False)
+    return ps, res.psi
+
+
+def c1_spinor(device):
+    """The 512^2 Raman-coupled instance, initial state as described above."""
+    from scipy.interpolate import RegularGridInterpolator
+    from spinor_gpe.pspinor import pspinor as spin
+    ps0, psi0 = ground_state_256(device)
+    ps = spin.PSpinor('c1', omeg={'x': W50, 'y': W50, 'z': 40 * W50},
+                      g_sc={'uu': 1.0, 'dd': 1.0, 'ud': 0.995}, mesh_points=(512, 512),
+                      r_sizes=(24, 24), atom_num=1e4, pop_frac=(0.5, 0.5))
+    scale = 3.0 / (np.sqrt(2) * np.pi / WAVEL * ps.a_x)
+    ps.coupling_setup(wavel=WAVEL, scale=scale, kin_shift=False)
+    ps.coupling_uniform(2.0 * ps.EL_recoil)
+    ps.detuning_uniform(0.5 * ps.EL_recoil)
+    ps.rot_coupling = False
+    pts = np.stack([ps.space['y_mesh'].ravel(), ps.space['x_mesh'].ravel()], axis=1)
+    rng = np.random.default_rng(1)
+    psi = []
+    for p in psi0:
+        f = [RegularGridInterpolator((ps0.space['y'], ps0.space['x']), part, bounds_error=False,
+                                     fill_value=0.0) for part in (p.real, p.imag)]
+        v = (f[0](pts) + 1j * f[1](pts)).reshape(512, 512)
+        psi.append(v * np.exp(1j * rng.normal(0.0, 0.05, (512, 512))))
+    tot = sum(np.sum(np.abs(p) ** 2) for p in psi) * ps.space['dv_r']
+    ps.psi = [p * np.sqrt(ps.atom_num / tot) for p in psi]
+    c = ps.space['dv_r'] / (2 * np.pi)
+    ps.psik = [np.fft.fftshift(np.fft.fftn(p) * c) for p in ps.psi]
+    return ps
+
+
+def time_engine(ps, device, precision, steps, warmup, block, prof_steps):
+    from spinor_gpe.pspinor.tensor_propagator import TensorPropagator
+    prop = TensorPropagator(ps, 1e-3, steps, device, time='real', precision=precision,
+                            block_steps=block, progress=False)
+    plan = prop._plan  # pylint: disable=protected-access
+    plan.run(warmup, block=block)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan.run(steps, block=block)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    col_ms, row_ms, n_col, n_row = plan.profile_steps(prof_steps)
+    atoms = sum(float(torch.sum(torch.abs(p) ** 2).item()) for p in prop.psik) * ps.space['dv_k']
+    return {'elapsed': elapsed, 'col_ms': col_ms / n_col, 'row_ms': row_ms / n_row,
+            'tuning': plan.tuning(), 'atoms': atoms}
+
+
+def cpu_baseline(ps, budget_s):
+    """Time the CPU oracle's full_step() on the same workload (bounded sample)."""
+    from oracle.reference_path import OraclePropagator
+    prop = OraclePropagator(ps, 1e-3, time='real')
+    prop.full_step()
+    times = []
+    t_start = time.perf_counter()
+    while time.perf_counter() - t_start < budget_s or len(times) < 3:
+        t0 = time.perf_counter()
+        prop.full_step()
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    return {'value': 1.0 / med, 'unit': 'split-steps/s', 'cores': torch.get_num_threads(),
+            'kind': 'port',
+            'sample': (f"512^2 Raman workload, complex128, {len(times)} timed full_step() calls "
+                       f"of the torch-CPU oracle (median {med * 1e3:.1f} ms)")}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split('\n')[0])
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20000)
+    ap.add_argument('--warmup', type=int, default=300)
+    ap.add_argument('--block', type=int, default=100, help='full steps per captured graph')
+    ap.add_argument('--profile-steps', type=int, default=200)
+    ap.add_argument('--cpu-budget', type=float, default=15.0, help='seconds of CPU baseline sampling')
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this engine measures one GPU; --gpus {args.gpus} is not supported "
+              "(instances shard without communication; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    if not torch.cuda.is_available():
+        print("bench.py: no HIP device available", file=sys.stderr)
+        sys.exit(1)
+    os.environ.setdefault('SPINOR_GPE_DATA', tempfile.mkdtemp(prefix='sgpe_bench_'))
+    device = 'cuda:0'
+    log("building the 512^2 initial state (256^2 ground state, 4000 imaginary steps)")
+    ps = c1_spinor(device)
+    n_pts = 512 * 512
+    results = {}
+    for prec in ('c128', 'c64'):
+        log(f"timing {prec}: warmup {args.warmup}, steps {args.steps}")
+        results[prec] = time_engine(ps, device, prec, args.steps, args.warmup, args.block,
+                                    args.profile_steps)
+    r = results['c128']
+    sps = args.steps / r['elapsed']
+    bytes_c = {'c128': 16, 'c64': 8}
+
+    def roofline(res, prec):
+        dom = 'row_pass' if res['row_ms'] >= res['col_ms'] else 'col_pass'
+        ms = max(res['row_ms'], res['col_ms'])
+        alg = 2 * 2 * n_pts * bytes_c[prec]
+        achieved = alg / (ms * 1e-3) / 1e9
+        return {'bound': 'hbm', 'kernel': dom, 'achieved': round(achieved, 1),
+                'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4),
+                'traffic': None, 'avg_us': round(ms * 1e3, 3), 'algorithmic_bytes': alg,
+                'col_pass_us': round(res['col_ms'] * 1e3, 3), 'row_pass_us': round(res['row_ms'] * 1e3, 3)}
+
+    r64 = results['c64']
+    sps64 = args.steps / r64['elapsed']
+    line = {
+        'metric': METRIC, 'value': round(sps, 2), 'unit': 'split-steps/s', 'n_gpus': 1,
+        'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': round(r['elapsed'] / args.steps * 1e3, 5),
+        'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': round(sps / TITAN_V_512, 2),
+        'dtype': 'c128 (f64)', 'data': 'synthetic (seeded Thomas-Fermi/ground-state inputs)',
+        'config': {'workload': '512x512 real-time Raman-coupled GPE, 1 instance, c128',
+                   'grid': [512, 512], 'r_sizes': [24, 24], 'dt': 1e-3, 'omega_ER': 2.0,
+                   'delta_ER': 0.5, 'kL': 3.0, 'frame': 'lab', 'atoms': 1e4,
+                   'graph_block_steps': args.block, 'tuning_row_e_rows_col_e_w': list(r['tuning']),
+                   'baseline': 'Titan V 149.1 split-steps/s at 512^2, complex128 '
+                               '(reference bench_data/TitanV_cuda_step.npz)'},
+        'grid_point_steps_per_s': round(sps * n_pts, 1),
+        'roofline': roofline(r, 'c128'),
+        'c64': {'value': round(sps64, 2), 'ms_per_step': round(r64['elapsed'] / args.steps * 1e3, 5),
+                'grid_point_steps_per_s': round(sps64 * n_pts, 1), 'roofline': roofline(r64, 'c64'),
+                'dtype': 'c64 (f32 arithmetic, f64 reductions)'},
+        'atoms_after_run': {'c128': r['atoms'], 'c64': r64['atoms']},
+    }
+    if not args.no_cpu_baseline:
+        log(f"CPU baseline: oracle full_step() for ~{args.cpu_budget:.0f} s")
+        line['cpu_baseline'] = cpu_baseline(ps, args.cpu_budget)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == '__main__':
+    main()

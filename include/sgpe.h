@@ -131,11 +131,17 @@ int sgpe_plan_set_slot(sgpe_plan* plan, int slot, const sgpe_inst_desc* desc, vo
  * run; physics tables are kept. Synchronizes the stream. */
 int sgpe_plan_set_state(sgpe_plan* plan, int slot, void* psik);
 
-/* Start a run for one slot: device buffer pops (2 doubles per full step,
- * capacity cap steps; NULL for none) receives the per-step populations of
- * prop_loop (tensor_propagator.py:173-194); conv_tol > 0 enables the
- * energy-based convergence test of sgpe_check_convergence. */
+/* Start a run for one slot (and activate it): device buffer pops (2 doubles
+ * per full step, capacity cap steps; NULL for none) receives the per-step
+ * populations of prop_loop (tensor_propagator.py:173-194); conv_tol > 0
+ * enables the convergence test of sgpe_energy, which deactivates the slot
+ * once |E_n - E_prev| / |E_n| per full step <= conv_tol. */
 int sgpe_plan_set_run(sgpe_plan* plan, int slot, double* pops, int cap, double conv_tol);
+
+/* Per-slot run state, read back to HOST memory (synchronizes the stream):
+ * out3[3*slot + {0,1,2}] = {active, converged, full steps in this run};
+ * energy8 (optional) = the last sgpe_energy results per slot. */
+int sgpe_plan_status(sgpe_plan* plan, int* out3, double* energy8);
 
 /* Enable (1) or disable (0) a slot; disabled slots are skipped by every
  * kernel (their blocks exit at once). */
@@ -162,6 +168,14 @@ int sgpe_full_steps(sgpe_plan* plan, int n_steps);
 /* Same as sgpe_full_steps(plan, steps_per_graph * n_graphs), issued as
  * n_graphs launches of one captured hipGraph of steps_per_graph steps. */
 int sgpe_graph_steps(sgpe_plan* plan, int steps_per_graph, int n_graphs);
+
+/* Instrumented form of sgpe_full_steps for measurement: the same launches,
+ * issued eagerly with a hipEvent recorded after each kernel on the context
+ * stream, then synchronized. out4 (host) = {column-pass ms, row-pass ms,
+ * column launches, row launches}. It serves the per-step timing methodology
+ * of the reference's benchmark (spinor_gpe/benchmarks/benchmark_prop.py:
+ * 69-79), which times prop.full_step() calls. */
+int sgpe_profile_full_steps(sgpe_plan* plan, int n_steps, double* out4);
 
 /* Bring every active slot's state to normalized k-space psik (closing
  * kinetic half-step + tensor_propagator.py:271) and record the last step's

@@ -11,7 +11,7 @@
 #include <vector>
 
 #include "../../include/sgpe.h"
-#include "sgpe_kernels.hip"
+#include "sgpe_registry.h"
 
 using namespace sgpe;
 
@@ -65,71 +65,10 @@ static void twiddles_f64(int n, std::vector<double>& re, std::vector<double>& im
 
 // ------------------------------------------------------- kernel registry
 
-typedef void (*ColFn)(dim3, dim3, size_t, hipStream_t, const InstDev*, RunDev*, const void*, int, int, int);
-typedef void (*RowFn)(dim3, dim3, size_t, hipStream_t, const InstDev*, RunDev*, const void*, int, int, int);
-typedef void (*FRowFn)(dim3, dim3, size_t, hipStream_t, const void*, void*, int, int, double, const void*);
-typedef void (*FColFn)(dim3, dim3, size_t, hipStream_t, const void*, void*, int, int, const void*);
-
-template <typename R, int N, int E, int W>
-static void col_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const InstDev* i, RunDev* r, const void* tw, int m,
-                       int dp, int dc) {
-  hipLaunchKernelGGL((col_pass_kernel<R, N, E, W>), g, b, l, s, i, r, (const typename Cx<R>::T*)tw, m, dp, dc);
-}
-template <typename R, int N, int E, int ROWS>
-static void row_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const InstDev* i, RunDev* r, const void* tw,
-                       int dur, int first, int colw) {
-  hipLaunchKernelGGL((row_pass_kernel<R, N, E, ROWS>), g, b, l, s, i, r, (const typename Cx<R>::T*)tw, dur, first,
-                     colw);
-}
-template <typename R, int N, int E, int ROWS>
-static void frow_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const void* in, void* out, int inv, int shift,
-                        double scale, const void* tw) {
-  typedef typename Cx<R>::T C;
-  hipLaunchKernelGGL((fft_rows_kernel<R, N, E, ROWS>), g, b, l, s, (const C*)in, (C*)out, inv, shift, scale,
-                     (const C*)tw);
-}
-template <typename R, int N, int E, int W>
-static void fcol_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const void* in, void* out, int nx, int inv,
-                        const void* tw) {
-  typedef typename Cx<R>::T C;
-  hipLaunchKernelGGL((fft_cols_kernel<R, N, E, W>), g, b, l, s, (const C*)in, (C*)out, nx, inv, (const C*)tw);
-}
-
-struct KEntry {
-  int prec, n, e, p;  // p = W (columns) or ROWS (rows)
-  void* launch;
-  const void* kernel;
-};
-
-#define COL_E(R, PR, N, E, W) {PR, N, E, W, (void*)&col_launch<R, N, E, W>, (const void*)&col_pass_kernel<R, N, E, W>},
-#define ROW_E(R, PR, N, E, RW) \
-  {PR, N, E, RW, (void*)&row_launch<R, N, E, RW>, (const void*)&row_pass_kernel<R, N, E, RW>},
-#define FROW_E(R, PR, N, E, RW) \
-  {PR, N, E, RW, (void*)&frow_launch<R, N, E, RW>, (const void*)&fft_rows_kernel<R, N, E, RW>},
-#define FCOL_E(R, PR, N, E, W) {PR, N, E, W, (void*)&fcol_launch<R, N, E, W>, (const void*)&fft_cols_kernel<R, N, E, W>},
-
-// Column shapes (N, E, W) and row shapes (N, E, ROWS) compiled for both precisions.
-#define COL_SHAPES(X, R, PR)                                                                              \
-  X(R, PR, 64, 8, 8) X(R, PR, 64, 8, 16) X(R, PR, 128, 8, 4) X(R, PR, 128, 8, 8) X(R, PR, 128, 8, 16)      \
-  X(R, PR, 256, 8, 4) X(R, PR, 256, 8, 8) X(R, PR, 256, 16, 8) X(R, PR, 512, 8, 2) X(R, PR, 512, 8, 4)    \
-  X(R, PR, 512, 8, 8) X(R, PR, 512, 16, 4) X(R, PR, 512, 16, 8) X(R, PR, 1024, 8, 2) X(R, PR, 1024, 8, 4) \
-  X(R, PR, 1024, 16, 4) X(R, PR, 1024, 16, 8) X(R, PR, 2048, 8, 2) X(R, PR, 2048, 8, 4)                 \
-  X(R, PR, 2048, 16, 2) X(R, PR, 2048, 16, 4) X(R, PR, 4096, 16, 2) X(R, PR, 4096, 16, 4)
-#define ROW_SHAPES(X, R, PR)                                                                                \
-  X(R, PR, 64, 8, 8) X(R, PR, 64, 8, 16) X(R, PR, 128, 8, 4) X(R, PR, 128, 8, 8) X(R, PR, 256, 8, 2)        \
-  X(R, PR, 256, 8, 4) X(R, PR, 256, 16, 4) X(R, PR, 512, 8, 1) X(R, PR, 512, 8, 2) X(R, PR, 512, 16, 2)     \
-  X(R, PR, 512, 16, 4) X(R, PR, 1024, 8, 1) X(R, PR, 1024, 16, 1) X(R, PR, 1024, 16, 2) X(R, PR, 2048, 8, 1) \
-  X(R, PR, 2048, 16, 1) X(R, PR, 4096, 16, 1)
-
-static const KEntry k_cols[] = {COL_SHAPES(COL_E, double, SGPE_C128) COL_SHAPES(COL_E, float, SGPE_C64)};
-static const KEntry k_rows[] = {ROW_SHAPES(ROW_E, double, SGPE_C128) ROW_SHAPES(ROW_E, float, SGPE_C64)};
-static const KEntry k_frows[] = {ROW_SHAPES(FROW_E, double, SGPE_C128) ROW_SHAPES(FROW_E, float, SGPE_C64)};
-static const KEntry k_fcols[] = {COL_SHAPES(FCOL_E, double, SGPE_C128) COL_SHAPES(FCOL_E, float, SGPE_C64)};
-
-template <size_t K>
-static const KEntry* find_entry(const KEntry (&tab)[K], int prec, int n, int e, int p) {
-  for (size_t i = 0; i < K; ++i)
-    if (tab[i].prec == prec && tab[i].n == n && tab[i].e == e && tab[i].p == p) return &tab[i];
+static const KEntry* find_entry(int kind, int prec, int n, int e, int p) {
+  const KTable t = prec == SGPE_C128 ? registry_f64(kind) : registry_f32(kind);
+  for (int i = 0; i < t.count; ++i)
+    if (t.entries[i].n == n && t.entries[i].e == e && t.entries[i].p == p) return &t.entries[i];
   return nullptr;
 }
 
@@ -245,6 +184,10 @@ struct sgpe_plan {
   int last_dur = DUR_OUT;  // duration of the last sub-step issued (closing K pending)
   std::map<int, hipGraphExec_t> graphs;
   std::map<int, hipGraph_t> graph_defs;
+  void* scratch = nullptr;  // [nslots][2][ny][nx] for energy evaluation
+  const KEntry* ecol = nullptr;
+  const KEntry* erow = nullptr;
+  int e_col_e = 8, e_col_w = 4, e_row_e = 8, e_row_rows = 1;
 };
 
 static void drop_graphs(sgpe_plan* p) {
@@ -255,8 +198,8 @@ static void drop_graphs(sgpe_plan* p) {
 }
 
 static int plan_select(sgpe_plan* p, int row_e, int row_rows, int col_e, int col_w) {
-  const KEntry* r = find_entry(k_rows, p->prec, p->nx, row_e, row_rows);
-  const KEntry* c = find_entry(k_cols, p->prec, p->ny, col_e, col_w);
+  const KEntry* r = find_entry(K_ROW, p->prec, p->nx, row_e, row_rows);
+  const KEntry* c = find_entry(K_COL, p->prec, p->ny, col_e, col_w);
   if (!r || !c) return fail(SGPE_E_UNSUPPORTED, "FFT shape not compiled for this grid/precision");
   if (p->ny % row_rows || p->nx % col_w) return fail(SGPE_E_UNSUPPORTED, "tile does not divide the grid");
   const size_t row_lds = (size_t)row_rows * 2 * p->nx * csize(p->prec);
@@ -312,6 +255,21 @@ static void issue_sync(sgpe_plan* p, hipStream_t s) {
   hipLaunchKernelGGL(set_form_kernel, dim3((p->nslots + 63) / 64), dim3(64), 0, s, p->d_run, p->nslots,
                      (int)FORM_KSPACE);
   p->last_dur = DUR_OUT;
+}
+
+// Normalized k-space state, then the energy passes and the per-slot
+// reduction / convergence test (all stream-ordered kernels, graph-safe).
+static void issue_energy(sgpe_plan* p, hipStream_t s, double* out) {
+  issue_sync(p, s);
+  const size_t cs = csize(p->prec);
+  const size_t cl = std::max<size_t>((size_t)p->e_col_w * p->ny * cs, 256);
+  const size_t rl = std::max<size_t>((size_t)p->e_row_rows * 2 * p->nx * cs, 256);
+  dim3 cg(p->nx / p->e_col_w, 2, p->nslots), cb(p->e_col_w * (p->ny / p->e_col_e));
+  dim3 rg(p->ny / p->e_row_rows, 1, p->nslots), rb(p->e_row_rows * (p->nx / p->e_row_e));
+  ((EFn)p->ecol->launch)(cg, cb, cl, s, p->d_inst, p->d_run, p->tw_y);
+  ((EFn)p->erow->launch)(rg, rb, rl, s, p->d_inst, p->d_run, p->tw_x);
+  hipLaunchKernelGGL(energy_final_kernel, dim3(p->nslots), dim3(256), 0, s, p->d_inst, p->d_run,
+                     p->nx / p->e_col_w, p->ny / p->e_row_rows, out);
 }
 
 // Host-side construction of one slot's InstDev and device buffer.
@@ -382,6 +340,8 @@ static int plan_load_slot(sgpe_plan* p, int slot, const sgpe_inst_desc* d, void*
   h.fft_c = d->dx * d->dy / (2.0 * M_PI);
   h.dt_sub[0] = dts[0];
   h.dt_sub[1] = dts[1];
+  h.omega = d->coupling == SGPE_COUPLING_UNIFORM ? d->omega : 0.0;
+  h.estride = 2 * nx + ny;
 
   Staging st;
   size_t o_kx[2][2], o_ky[2][2], o_px[2][2], o_py[2][2], o_eth = 0;
@@ -431,6 +391,7 @@ static int plan_load_slot(sgpe_plan* p, int slot, const sgpe_inst_desc* d, void*
     }
   const size_t o_part = st.add((size_t)PART_SLOTS * 2 * nx * sizeof(double));
   const size_t o_epart = st.add((size_t)EPART_SLOTS * (2 * nx + ny) * sizeof(double));
+  h.scratch = p->scratch ? (char*)p->scratch + (size_t)slot * 2 * ny * nx * cs : nullptr;
 
   // Phase tables in float64, rounded once to the plan precision.
   for (int dd = 0; dd < 2; ++dd) {
@@ -588,8 +549,8 @@ int sgpe_fft2(sgpe_ctx* ctx, int precision, int ny, int nx, int n_arrays, void* 
   int re, rr, ce, cw;
   default_row(precision, nx, &re, &rr);
   default_col(precision, ny, &ce, &cw);
-  const KEntry* fr = find_entry(k_frows, precision, nx, re, rr);
-  const KEntry* fc = find_entry(k_fcols, precision, ny, ce, cw);
+  const KEntry* fr = find_entry(K_FROW, precision, nx, re, rr);
+  const KEntry* fc = find_entry(K_FCOL, precision, ny, ce, cw);
   if (!fr || !fc) return fail(SGPE_E_UNSUPPORTED, "FFT shape not compiled");
   const size_t rl = std::max<size_t>((size_t)rr * nx * csize(precision), 256);
   const size_t cl = std::max<size_t>((size_t)cw * ny * csize(precision), 256);
@@ -720,6 +681,7 @@ void sgpe_plan_destroy(sgpe_plan* p) {
   drop_graphs(p);
   for (auto& s : p->slots)
     if (s.buf) (void)hipFree(s.buf);
+  if (p->scratch) (void)hipFree(p->scratch);
   if (p->d_inst) (void)hipFree(p->d_inst);
   if (p->d_run) (void)hipFree(p->d_run);
   delete p;
@@ -750,8 +712,24 @@ int sgpe_plan_set_run(sgpe_plan* p, int slot, double* pops, int cap, double conv
   if (!p || slot < 0 || slot >= p->nslots || cap < 0) return fail(SGPE_E_ARG, "bad arguments");
   HIP_TRY(hipSetDevice(p->ctx->device));
   hipLaunchKernelGGL(config_run_kernel, dim3(1), dim3(64), 0, p->ctx->stream, p->d_run, slot, pops, pops ? cap : 0, 1,
-                     -1, -1, conv_tol);
+                     1, -1, conv_tol);
   HIP_TRY(hipGetLastError());
+  return SGPE_OK;
+}
+
+int sgpe_plan_status(sgpe_plan* p, int* out3, double* energy8) {
+  if (!p || !out3) return fail(SGPE_E_ARG, "bad arguments");
+  HIP_TRY(hipSetDevice(p->ctx->device));
+  std::vector<RunDev> h(p->nslots);
+  HIP_TRY(hipMemcpyAsync(h.data(), p->d_run, p->nslots * sizeof(RunDev), hipMemcpyDeviceToHost, p->ctx->stream));
+  HIP_TRY(hipStreamSynchronize(p->ctx->stream));
+  for (int i = 0; i < p->nslots; ++i) {
+    out3[3 * i + 0] = h[i].active;
+    out3[3 * i + 1] = h[i].converged;
+    out3[3 * i + 2] = h[i].step;
+    if (energy8)
+      for (int j = 0; j < 8; ++j) energy8[8 * i + j] = h[i].energy[j];
+  }
   return SGPE_OK;
 }
 
@@ -833,6 +811,55 @@ int sgpe_graph_steps(sgpe_plan* p, int k, int n_graphs) {
   return SGPE_OK;
 }
 
+int sgpe_profile_full_steps(sgpe_plan* p, int n_steps, double* out4) {
+  int rc = plan_ready(p);
+  if (rc) return rc;
+  if (n_steps <= 0 || !out4) return fail(SGPE_E_ARG, "bad arguments");
+  HIP_TRY(hipSetDevice(p->ctx->device));
+  hipStream_t s = p->ctx->stream;
+  const int n_launch = 6 * n_steps;
+  std::vector<hipEvent_t> ev(n_launch + 1, nullptr);
+  for (auto& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      for (auto& f : ev)
+        if (f) (void)hipEventDestroy(f);
+      return fail(SGPE_E_HIP, "event creation failed");
+    }
+  // Same launches as sgpe_full_steps, with an event between consecutive
+  // kernels: launch 2i is a column pass, 2i+1 a row pass.
+  (void)hipEventRecord(ev[0], s);
+  int k = 0;
+  for (int i = 0; i < n_steps; ++i) {
+    const int durs[3] = {DUR_OUT, DUR_IN, DUR_OUT};
+    for (int j = 0; j < 3; ++j) {
+      launch_col(p, s, COL_MID, p->last_dur, durs[j]);
+      (void)hipEventRecord(ev[++k], s);
+      launch_row(p, s, durs[j], j == 0);
+      (void)hipEventRecord(ev[++k], s);
+      p->last_dur = durs[j];
+    }
+  }
+  hipError_t le = hipGetLastError();
+  hipError_t se = hipStreamSynchronize(s);
+  double col = 0.0, row = 0.0;
+  for (int i = 0; i < n_launch && le == hipSuccess && se == hipSuccess; ++i) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) != hipSuccess) break;
+    if (i % 2 == 0)
+      col += ms;
+    else
+      row += ms;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (le != hipSuccess) return fail(SGPE_E_HIP, std::string("launch failed: ") + hipGetErrorString(le));
+  if (se != hipSuccess) return fail(SGPE_E_HIP, std::string("synchronize failed: ") + hipGetErrorString(se));
+  out4[0] = col;
+  out4[1] = row;
+  out4[2] = 3.0 * n_steps;
+  out4[3] = 3.0 * n_steps;
+  return SGPE_OK;
+}
+
 int sgpe_plan_sync(sgpe_plan* p) {
   int rc = plan_ready(p);
   if (rc) return rc;
@@ -843,10 +870,33 @@ int sgpe_plan_sync(sgpe_plan* p) {
 }
 
 int sgpe_energy(sgpe_plan* p, double* out) {
-  (void)out;
   int rc = plan_ready(p);
   if (rc) return rc;
-  return fail(SGPE_E_UNSUPPORTED, "energy evaluation is not available in this build");
+  HIP_TRY(hipSetDevice(p->ctx->device));
+  hipStream_t s = p->ctx->stream;
+  const size_t cs = csize(p->prec);
+  if (!p->scratch) {
+    // Lazily allocated work state, one per slot; every slot's descriptor is
+    // re-uploaded with its pointer.
+    const size_t per = (size_t)2 * p->ny * p->nx * cs;
+    if (hipMalloc(&p->scratch, per * p->nslots) != hipSuccess) {
+      p->scratch = nullptr;
+      return fail(SGPE_E_NOMEM, "energy work buffer allocation failed");
+    }
+    for (int i = 0; i < p->nslots; ++i) p->h_inst[i].scratch = (char*)p->scratch + i * per;
+    HIP_TRY(hipMemcpyAsync(p->d_inst, p->h_inst.data(), p->nslots * sizeof(InstDev), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    default_col(p->prec, p->ny, &p->e_col_e, &p->e_col_w);
+    default_row(p->prec, p->nx, &p->e_row_e, &p->e_row_rows);
+    p->ecol = find_entry(K_ECOL, p->prec, p->ny, p->e_col_e, p->e_col_w);
+    p->erow = find_entry(K_EROW, p->prec, p->nx, p->e_row_e, p->e_row_rows);
+    if (!p->ecol || !p->erow) return fail(SGPE_E_UNSUPPORTED, "energy kernels not compiled for this grid");
+    if ((rc = set_lds_attr(p->ecol->kernel, (size_t)p->e_col_w * p->ny * cs))) return rc;
+    if ((rc = set_lds_attr(p->erow->kernel, (size_t)p->e_row_rows * 2 * p->nx * cs))) return rc;
+  }
+  issue_energy(p, s, out);
+  HIP_TRY(hipGetLastError());
+  return SGPE_OK;
 }
 
 }  // extern "C"

@@ -46,7 +46,9 @@ enum {
 struct InstDev {
   void* psi;        // [2][ny][nx] complex state
   double* part;     // column-pass partial sums, [PART_SLOTS][ntile]
-  double* epart;    // energy partial sums
+  double* epart;    // energy partial sums, [EPART_SLOTS][estride]
+  void* scratch;    // [2][ny][nx] work state for energy evaluation (or null)
+  int estride;      // entries per energy partial slot
   int ny, nx;
   int imag;         // 1: imaginary time (t_step = -i dt), 0: real time
   int coupling;     // 0: none, 1: uniform Omega, 2: dense Omega array
@@ -62,6 +64,7 @@ struct InstDev {
   const void* py[2][2];
   const double* v_dense[2];      // non-null: dense V_s (f64)
   // Coupling C = [[alpha, beta e^{-i theta}], [beta e^{+i theta}, alpha]]
+  double omega;                  // uniform Omega (coupling == 1)
   double c_alpha[2];             // uniform Omega: per duration
   double c_beta_re[2], c_beta_im[2];
   const double* omega_dense;     // coupling == 2: dense Omega (f64)
@@ -91,6 +94,8 @@ struct RunDev {
 };
 
 enum { PART_S = 0, PART_NRM = 1, PART_SLOTS = 2 };
-enum { EPART_KIN = 0, EPART_S = 1, EPART_POT = 2, EPART_INT = 3, EPART_CPL = 4, EPART_SLOTS = 5 };
+// Energy partial sums: column-pass slots (KIN, S) per column tile, row-pass
+// slots (POT, INT, CPL, NRM) per row block; each slot has estride entries.
+enum { EPART_KIN = 0, EPART_S = 1, EPART_POT = 2, EPART_INT = 3, EPART_CPL = 4, EPART_NRM = 5, EPART_SLOTS = 6 };
 
 }  // namespace sgpe

@@ -16,6 +16,8 @@
 // (tensor_propagator.py:271) is deferred: the next real-space renormalization
 // absorbs it, the per-step populations are ratios of the S sums, and the
 // explicit normalize kernel applies it when the state is handed back.
+#pragma once
+
 #include "sgpe_fft.h"
 
 namespace sgpe {
@@ -161,9 +163,11 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
         run.pops[2 * (step - 1) + 1] = P.atom_num * s[1] / tot;
       }
       run.step = step + 1;
-      run.form = FORM_HYBRID;
     }
   }
+  // The state leaves this pass in hybrid form (no other block of this kernel
+  // reads the flag; the next column pass does).
+  if (blockIdx.x == 0 && threadIdx.x == 0) run.form = FORM_HYBRID;
   const double lam2 = P.atom_num / (P.dv_r * (double)N * (double)ny * nrm[0]);
 
   const int row = threadIdx.x / T;
@@ -302,18 +306,18 @@ __global__ __launch_bounds__(256) void normalize_kernel(const InstDev* __restric
 
 // Marks the state as normalized k-space. Separate launch so that no block of
 // the normalize kernel can observe a half-updated flag.
-__global__ void set_form_kernel(RunDev* __restrict__ runs, int n, int form) {
+static __global__ void set_form_kernel(RunDev* __restrict__ runs, int n, int form) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n && runs[i].active) runs[i].form = form;
 }
 
-__global__ void set_active_kernel(RunDev* __restrict__ runs, int slot, int active) {
+static __global__ void set_active_kernel(RunDev* __restrict__ runs, int slot, int active) {
   if (threadIdx.x == 0) runs[slot].active = active;
 }
 
 // Run-state (re)configuration for one slot; launched instead of copying so
 // that it is ordered on the stream and graph-safe.
-__global__ void config_run_kernel(RunDev* __restrict__ runs, int slot, double* pops, int cap, int reset_step,
+static __global__ void config_run_kernel(RunDev* __restrict__ runs, int slot, double* pops, int cap, int reset_step,
                                   int active, int form, double conv_tol) {
   if (threadIdx.x != 0) return;
   RunDev& r = runs[slot];
@@ -329,6 +333,179 @@ __global__ void config_run_kernel(RunDev* __restrict__ runs, int slot, double* p
   if (active >= 0) r.active = active;
   if (form >= 0) r.form = form;
   r.conv_tol = conv_tol;
+}
+
+// ------------------------------------------------------------------ energy
+// GP energy functional of a normalized k-space state, float64 sums:
+//   E_kin = dv_k sum_s sum_k T_s |psik_s|^2
+//   E_pot = dv_r sum_s sum_r V_s n_s, E_int = dv_r/2 sum (g_uu n0^2 + g_dd n1^2 + 2 g_ud n0 n1)
+//   E_cpl = dv_r sum Omega Re(psi0* psi1 e^{-i theta}),  mu = (E_kin + E_pot + E_cpl + 2 E_int)/N
+// with real space from chi = IDFT2(psik): n = |chi|^2 / (Nx Ny c)^2 (the
+// (-1)^(x+y) of the shift cancels in every term).
+
+// Column part: kinetic and norm sums in k-space, then IDFT_y into scratch.
+template <typename R, int N, int E, int W>
+__global__ __launch_bounds__(W*(N / E)) void energy_col_kernel(const InstDev* __restrict__ insts,
+                                                               const RunDev* __restrict__ runs,
+                                                               const typename Cx<R>::T* __restrict__ tw) {
+  typedef typename Cx<R>::T C;
+  typedef LdsView<W, 1 << 30, 0> V;
+  constexpr int T = N / E;
+  extern __shared__ double smem[];
+  const int inst = blockIdx.z;
+  if (!runs[inst].active) return;
+  const InstDev& P = insts[inst];
+  const int nx = P.nx;
+  const int comp = blockIdx.y;
+  const int tile = blockIdx.x;
+  const int c = threadIdx.x % W;
+  const int t = threadIdx.x / W;
+  const int jx = tile * W + c;
+  C* lds = reinterpret_cast<C*>(smem) + c;
+  const C* col = reinterpret_cast<const C*>(P.psi) + (size_t)comp * N * nx + jx;
+  C* out = reinterpret_cast<C*>(P.scratch) + (size_t)comp * N * nx + jx;
+  C a[E];
+#pragma unroll
+  for (int j = 0; j < E; ++j) a[j] = col[(size_t)(t + j * T) * nx];
+  const double* td = P.t_dense[comp];
+  const double tx = td ? 0.0 : P.tx[comp][jx];
+  double kin = 0.0, s = 0.0;
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const int jy = t + j * T;
+    const double tv = td ? td[(size_t)jy * nx + jx] : tx + P.ty[comp][jy];
+    const double m = norm2d<R>(a[j]);
+    kin += tv * m;
+    s += m;
+  }
+  fft_run<N, E, true, V>(a, t, tw, lds);
+#pragma unroll
+  for (int j = 0; j < E; ++j) out[(size_t)(t + j * T) * nx] = a[j];
+  double v[2] = {kin, s};
+  block_sum<2>(v, smem);
+  if (threadIdx.x == 0) {
+    const int nt = nx / W;
+    P.epart[EPART_KIN * P.estride + comp * nt + tile] = v[0];
+    P.epart[EPART_S * P.estride + comp * nt + tile] = v[1];
+  }
+}
+
+// Row part: IDFT_x of scratch rows, real-space sums (nothing is written back).
+template <typename R, int N, int E, int ROWS>
+__global__ __launch_bounds__(ROWS*(N / E)) void energy_row_kernel(const InstDev* __restrict__ insts,
+                                                                  const RunDev* __restrict__ runs,
+                                                                  const typename Cx<R>::T* __restrict__ tw) {
+  typedef typename Cx<R>::T C;
+  typedef LdsView<1, 1 << 30, 0> V;
+  constexpr int T = N / E;
+  extern __shared__ double smem[];
+  const int inst = blockIdx.z;
+  if (!runs[inst].active) return;
+  const InstDev& P = insts[inst];
+  const int ny = P.ny;
+  const int row = threadIdx.x / T;
+  const int t = threadIdx.x % T;
+  const int y = blockIdx.x * ROWS + row;
+  const C* r0 = reinterpret_cast<const C*>(P.scratch) + (size_t)y * N;
+  const C* r1 = r0 + (size_t)ny * N;
+  C* lds0 = reinterpret_cast<C*>(smem) + (size_t)row * 2 * N;
+  C* lds1 = lds0 + N;
+  C a0[E], a1[E];
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    a0[j] = r0[t + j * T];
+    a1[j] = r1[t + j * T];
+  }
+  fft_run2<N, E, true, V>(a0, a1, t, tw, lds0, lds1);
+  const double* vd0 = P.v_dense[0];
+  const double* vd1 = P.v_dense[1];
+  const double vy0 = vd0 ? 0.0 : P.vy[0][y];
+  const double vy1 = vd1 ? 0.0 : P.vy[1][y];
+  const C* eth = reinterpret_cast<const C*>(P.etheta);
+  double pot = 0.0, inter = 0.0, cpl = 0.0, nrm = 0.0;
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const int x = t + j * T;
+    const double m0 = norm2d<R>(a0[j]), m1 = norm2d<R>(a1[j]);
+    const double v0 = vd0 ? vd0[(size_t)y * N + x] : P.vx[0][x] + vy0;
+    const double v1 = vd1 ? vd1[(size_t)y * N + x] : P.vx[1][x] + vy1;
+    pot += v0 * m0 + v1 * m1;
+    inter += P.g_uu * m0 * m0 + P.g_dd * m1 * m1 + 2.0 * P.g_ud * m0 * m1;
+    nrm += m0 + m1;
+    if (P.coupling) {
+      const double om = P.coupling == 2 ? P.omega_dense[(size_t)y * N + x] : P.omega;
+      // Re(conj(a0) a1 e^{-i theta})
+      double cr = (double)a0[j].x * a1[j].x + (double)a0[j].y * a1[j].y;
+      double ci = (double)a0[j].x * a1[j].y - (double)a0[j].y * a1[j].x;
+      if (eth) {
+        const double er = (double)eth[x].x, ei = (double)eth[x].y;
+        cr = cr * er - ci * ei;
+      }
+      cpl += om * cr;
+    }
+  }
+  double v[4] = {pot, inter, cpl, nrm};
+  block_sum<4>(v, smem);
+  if (threadIdx.x == 0) {
+    P.epart[EPART_POT * P.estride + blockIdx.x] = v[0];
+    P.epart[EPART_INT * P.estride + blockIdx.x] = v[1];
+    P.epart[EPART_CPL * P.estride + blockIdx.x] = v[2];
+    P.epart[EPART_NRM * P.estride + blockIdx.x] = v[3];
+  }
+}
+
+// One block per slot: reduce the partials, write the energy terms and run
+// the convergence test |E - E_prev| / |E| per full step <= conv_tol (a
+// converged slot is deactivated, freezing its normalized k-space state).
+static __global__ __launch_bounds__(256) void energy_final_kernel(const InstDev* __restrict__ insts,
+                                                           RunDev* __restrict__ runs, int n_col_tiles,
+                                                           int n_row_blocks, double* __restrict__ out) {
+  __shared__ double scratch[32];
+  const int inst = blockIdx.x;
+  RunDev& run = runs[inst];
+  if (!run.active) return;
+  const InstDev& P = insts[inst];
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  const int ncol = 2 * n_col_tiles;
+  for (int i = threadIdx.x; i < ncol; i += blockDim.x) {
+    v[0] += P.epart[EPART_KIN * P.estride + i];
+    v[1] += P.epart[EPART_S * P.estride + i];
+  }
+  for (int i = threadIdx.x; i < n_row_blocks; i += blockDim.x) {
+    v[2] += P.epart[EPART_POT * P.estride + i];
+    v[3] += P.epart[EPART_INT * P.estride + i];
+    v[4] += P.epart[EPART_CPL * P.estride + i];
+    v[5] += P.epart[EPART_NRM * P.estride + i];
+  }
+  block_sum<6>(v, scratch);
+  if (threadIdx.x != 0) return;
+  const double kappa2 = P.atom_num / (P.dv_k * v[1]);
+  const double nn = (double)P.nx * (double)P.ny * P.fft_c;
+  const double a = kappa2 / (nn * nn);
+  const double e_kin = kappa2 * P.dv_k * v[0];
+  const double e_pot = P.dv_r * a * v[2];
+  const double e_int = 0.5 * P.dv_r * a * a * v[3];
+  const double e_cpl = P.dv_r * a * v[4];
+  const double atoms = P.dv_r * a * v[5];
+  const double e_tot = e_kin + e_pot + e_int + e_cpl;
+  const double mu = (e_kin + e_pot + e_cpl + 2.0 * e_int) / atoms;
+  double rel = -1.0;
+  if (run.n_checks > 0) {
+    const int steps = run.step - run.steps_prev;
+    rel = fabs(e_tot - run.e_prev) / fabs(e_tot) / (double)(steps > 0 ? steps : 1);
+    if (run.conv_tol > 0.0 && steps > 0 && rel <= run.conv_tol) {
+      run.converged = 1;
+      run.active = 0;
+    }
+  }
+  const double e[8] = {e_tot, e_kin, e_pot, e_int, e_cpl, mu, atoms, rel};
+  for (int i = 0; i < 8; ++i) {
+    run.energy[i] = e[i];
+    if (out) out[8 * inst + i] = e[i];
+  }
+  run.e_prev = e_tot;
+  run.steps_prev = run.step;
+  run.n_checks += 1;
 }
 
 // ------------------------------------------------- standalone primitives
@@ -437,7 +614,7 @@ __global__ __launch_bounds__(256) void norm_apply_kernel(typename Cx<R>::T* __re
 }
 
 // Final reduction of density_sums partials into out2[s] = vol * sum |psi_s|^2.
-__global__ __launch_bounds__(256) void pops_final_kernel(const double* __restrict__ part, int nblk, double vol,
+static __global__ __launch_bounds__(256) void pops_final_kernel(const double* __restrict__ part, int nblk, double vol,
                                                          double* __restrict__ out2) {
   __shared__ double scratch[16];
   double v[2] = {0.0, 0.0};

@@ -1,0 +1,122 @@
+// Registry of compiled kernel shapes. Each precision's instantiations live in
+// their own translation unit (sgpe_inst_f64.hip, sgpe_inst_f32.hip) so that
+// they compile in parallel; sgpe_api.hip looks shapes up here.
+#pragma once
+
+#include "sgpe_kernels.h"
+
+namespace sgpe {
+
+typedef void (*ColFn)(dim3, dim3, size_t, hipStream_t, const InstDev*, RunDev*, const void*, int, int, int);
+typedef void (*RowFn)(dim3, dim3, size_t, hipStream_t, const InstDev*, RunDev*, const void*, int, int, int);
+typedef void (*FRowFn)(dim3, dim3, size_t, hipStream_t, const void*, void*, int, int, double, const void*);
+typedef void (*FColFn)(dim3, dim3, size_t, hipStream_t, const void*, void*, int, int, const void*);
+typedef void (*EFn)(dim3, dim3, size_t, hipStream_t, const InstDev*, const RunDev*, const void*);
+
+struct KEntry {
+  int prec, n, e, p;  // p = W (columns per block) or ROWS (rows per block)
+  void* launch;
+  const void* kernel;
+};
+
+struct KTable {
+  const KEntry* entries;
+  int count;
+};
+
+// One table per kernel kind and precision: [kind][prec].
+enum { K_COL = 0, K_ROW = 1, K_FROW = 2, K_FCOL = 3, K_ECOL = 4, K_EROW = 5, K_KINDS = 6 };
+KTable registry_f64(int kind);
+KTable registry_f32(int kind);
+
+template <typename R, int N, int E, int W>
+void col_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const InstDev* i, RunDev* r, const void* tw, int m, int dp,
+                int dc) {
+  hipLaunchKernelGGL((col_pass_kernel<R, N, E, W>), g, b, l, s, i, r, (const typename Cx<R>::T*)tw, m, dp, dc);
+}
+template <typename R, int N, int E, int ROWS>
+void row_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const InstDev* i, RunDev* r, const void* tw, int dur,
+                int first, int colw) {
+  hipLaunchKernelGGL((row_pass_kernel<R, N, E, ROWS>), g, b, l, s, i, r, (const typename Cx<R>::T*)tw, dur, first,
+                     colw);
+}
+template <typename R, int N, int E, int ROWS>
+void frow_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const void* in, void* out, int inv, int shift, double scale,
+                 const void* tw) {
+  typedef typename Cx<R>::T C;
+  hipLaunchKernelGGL((fft_rows_kernel<R, N, E, ROWS>), g, b, l, s, (const C*)in, (C*)out, inv, shift, scale,
+                     (const C*)tw);
+}
+template <typename R, int N, int E, int W>
+void fcol_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const void* in, void* out, int nx, int inv, const void* tw) {
+  typedef typename Cx<R>::T C;
+  hipLaunchKernelGGL((fft_cols_kernel<R, N, E, W>), g, b, l, s, (const C*)in, (C*)out, nx, inv, (const C*)tw);
+}
+template <typename R, int N, int E, int W>
+void ecol_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const InstDev* i, const RunDev* r, const void* tw) {
+  hipLaunchKernelGGL((energy_col_kernel<R, N, E, W>), g, b, l, s, i, r, (const typename Cx<R>::T*)tw);
+}
+template <typename R, int N, int E, int ROWS>
+void erow_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const InstDev* i, const RunDev* r, const void* tw) {
+  hipLaunchKernelGGL((energy_row_kernel<R, N, E, ROWS>), g, b, l, s, i, r, (const typename Cx<R>::T*)tw);
+}
+
+}  // namespace sgpe
+
+#define SGPE_COL_E(R, PR, N, E, W) \
+  {PR, N, E, W, (void*)&sgpe::col_launch<R, N, E, W>, (const void*)&sgpe::col_pass_kernel<R, N, E, W>},
+#define SGPE_ROW_E(R, PR, N, E, RW) \
+  {PR, N, E, RW, (void*)&sgpe::row_launch<R, N, E, RW>, (const void*)&sgpe::row_pass_kernel<R, N, E, RW>},
+#define SGPE_FROW_E(R, PR, N, E, RW) \
+  {PR, N, E, RW, (void*)&sgpe::frow_launch<R, N, E, RW>, (const void*)&sgpe::fft_rows_kernel<R, N, E, RW>},
+#define SGPE_FCOL_E(R, PR, N, E, W) \
+  {PR, N, E, W, (void*)&sgpe::fcol_launch<R, N, E, W>, (const void*)&sgpe::fft_cols_kernel<R, N, E, W>},
+#define SGPE_ECOL_E(R, PR, N, E, W) \
+  {PR, N, E, W, (void*)&sgpe::ecol_launch<R, N, E, W>, (const void*)&sgpe::energy_col_kernel<R, N, E, W>},
+#define SGPE_EROW_E(R, PR, N, E, RW) \
+  {PR, N, E, RW, (void*)&sgpe::erow_launch<R, N, E, RW>, (const void*)&sgpe::energy_row_kernel<R, N, E, RW>},
+
+// Tunable shapes of the two split-step passes: columns (N, E, W), rows (N, E, ROWS).
+#define SGPE_COL_SHAPES(X, R, PR)                                                                         \
+  X(R, PR, 64, 8, 8) X(R, PR, 64, 8, 16) X(R, PR, 128, 8, 4) X(R, PR, 128, 8, 8) X(R, PR, 128, 8, 16)      \
+  X(R, PR, 256, 8, 4) X(R, PR, 256, 8, 8) X(R, PR, 256, 16, 8) X(R, PR, 512, 8, 2) X(R, PR, 512, 8, 4)    \
+  X(R, PR, 512, 8, 8) X(R, PR, 512, 16, 4) X(R, PR, 512, 16, 8) X(R, PR, 1024, 8, 2) X(R, PR, 1024, 8, 4) \
+  X(R, PR, 1024, 16, 4) X(R, PR, 1024, 16, 8) X(R, PR, 2048, 8, 2) X(R, PR, 2048, 8, 4)                 \
+  X(R, PR, 2048, 16, 2) X(R, PR, 2048, 16, 4) X(R, PR, 4096, 16, 2) X(R, PR, 4096, 16, 4)
+#define SGPE_ROW_SHAPES(X, R, PR)                                                                           \
+  X(R, PR, 64, 8, 8) X(R, PR, 64, 8, 16) X(R, PR, 128, 8, 4) X(R, PR, 128, 8, 8) X(R, PR, 256, 8, 2)        \
+  X(R, PR, 256, 8, 4) X(R, PR, 256, 16, 4) X(R, PR, 512, 8, 1) X(R, PR, 512, 8, 2) X(R, PR, 512, 16, 2)     \
+  X(R, PR, 512, 16, 4) X(R, PR, 1024, 8, 1) X(R, PR, 1024, 16, 1) X(R, PR, 1024, 16, 2) X(R, PR, 2048, 8, 1) \
+  X(R, PR, 2048, 16, 1) X(R, PR, 4096, 16, 1)
+
+// Default shapes (standalone FFTs and energy evaluation use only these);
+// they must match default_row / default_col in sgpe_api.hip.
+#define SGPE_DEF_ROWS(X, R, PR)                                                                     \
+  X(R, PR, 64, 8, 8) X(R, PR, 128, 8, 4) X(R, PR, 256, 8, 2) X(R, PR, 512, 8, 1) X(R, PR, 1024, 8, 1) \
+  X(R, PR, 2048, 8, 1) X(R, PR, 4096, 16, 1)
+#define SGPE_DEF_COLS_F64(X, R, PR)                                                                 \
+  X(R, PR, 64, 8, 8) X(R, PR, 128, 8, 8) X(R, PR, 256, 8, 4) X(R, PR, 512, 8, 4) X(R, PR, 1024, 8, 4) \
+  X(R, PR, 2048, 16, 2) X(R, PR, 4096, 16, 2)
+#define SGPE_DEF_COLS_F32(X, R, PR)                                                                 \
+  X(R, PR, 64, 8, 8) X(R, PR, 128, 8, 8) X(R, PR, 256, 8, 4) X(R, PR, 512, 8, 4) X(R, PR, 1024, 8, 4) \
+  X(R, PR, 2048, 8, 4) X(R, PR, 4096, 16, 4)
+
+#define SGPE_DEFINE_REGISTRY(FN, R, PR, DEFCOLS)                                                     \
+  namespace sgpe {                                                                                   \
+  static const KEntry k_col[] = {SGPE_COL_SHAPES(SGPE_COL_E, R, PR)};                                \
+  static const KEntry k_row[] = {SGPE_ROW_SHAPES(SGPE_ROW_E, R, PR)};                                \
+  static const KEntry k_frow[] = {SGPE_DEF_ROWS(SGPE_FROW_E, R, PR)};                                \
+  static const KEntry k_fcol[] = {DEFCOLS(SGPE_FCOL_E, R, PR)};                                      \
+  static const KEntry k_ecol[] = {DEFCOLS(SGPE_ECOL_E, R, PR)};                                      \
+  static const KEntry k_erow[] = {SGPE_DEF_ROWS(SGPE_EROW_E, R, PR)};                                \
+  KTable FN(int kind) {                                                                              \
+    switch (kind) {                                                                                  \
+      case K_COL: return {k_col, (int)(sizeof(k_col) / sizeof(KEntry))};                            \
+      case K_ROW: return {k_row, (int)(sizeof(k_row) / sizeof(KEntry))};                            \
+      case K_FROW: return {k_frow, (int)(sizeof(k_frow) / sizeof(KEntry))};                         \
+      case K_FCOL: return {k_fcol, (int)(sizeof(k_fcol) / sizeof(KEntry))};                         \
+      case K_ECOL: return {k_ecol, (int)(sizeof(k_ecol) / sizeof(KEntry))};                         \
+      default: return {k_erow, (int)(sizeof(k_erow) / sizeof(KEntry))};                            \
+    }                                                                                                \
+  }                                                                                                  \
+  }

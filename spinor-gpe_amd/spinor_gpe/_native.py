@@ -74,12 +74,14 @@ def _declare(lib):
         'sgpe_plan_set_state': (i, [vp, i, vp]),
         'sgpe_plan_set_run': (i, [vp, i, vp, i, d]),
         'sgpe_plan_set_active': (i, [vp, i, i]),
+        'sgpe_plan_status': (i, [vp, ip, _dp]),
         'sgpe_plan_tune': (i, [vp, i, i, i, i]),
         'sgpe_plan_get_tuning': (i, [vp, ip]),
         'sgpe_single_step': (i, [vp, i]),
         'sgpe_full_steps': (i, [vp, i]),
         'sgpe_graph_steps': (i, [vp, i, i]),
         'sgpe_plan_sync': (i, [vp]),
+        'sgpe_profile_full_steps': (i, [vp, i, _dp]),
         'sgpe_energy': (i, [vp, vp]),
         'sgpe_unwrap_phase_2d': (i, [_dp, _dp, i, i]),
     }

@@ -208,11 +208,33 @@ class Plan:
         if rem:
             self.full_steps(rem)
 
+    def profile_steps(self, n):
+        """Run ``n`` full steps eagerly with a hipEvent after every kernel.
+
+        Returns (column-pass ms, row-pass ms, column launches, row launches).
+        """
+        out = (ctypes.c_double * 4)()
+        self.ctx.bind_stream()
+        nat.check(nat.lib().sgpe_profile_full_steps(self.handle, int(n), out))
+        return tuple(out)
+
     def sync(self):
         self._call('sgpe_plan_sync')
 
+    def status(self):
+        """Host copy of the run state: (active, converged, steps) int array
+        of shape (n_slots, 3) and the last energies, float64 (n_slots, 8)."""
+        st = np.zeros((self.n_slots, 3), dtype=np.int32)
+        en = np.zeros((self.n_slots, 8), dtype=np.float64)
+        self._call('sgpe_plan_status', st.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                   en.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return st, en
+
     def energy(self, out=None):
-        """Spectral GP energy per slot: float64 tensor (n_slots, 8)."""
+        """GP energy terms per slot, float64 tensor (n_slots, 8):
+        E_tot, E_kin, E_pot, E_int, E_cpl, mu, N, relative change per step
+        since the previous evaluation (-1 for the first). Leaves the states
+        in normalized k-space."""
         if out is None:
             out = torch.empty((self.n_slots, 8), dtype=torch.float64, device=self.device)
         self._call('sgpe_energy', ctypes.c_void_p(out.data_ptr()))

@@ -9,7 +9,10 @@ The only device accepted is a HIP GPU (``device='cuda'``).
 
 Keyword-only additions: ``precision`` ('c128' default, or 'c64': complex64
 state and arithmetic, float64 reductions), ``block_steps`` (full steps per
-captured graph) and ``progress`` (tqdm bar, default on as in the reference).
+captured graph), ``progress`` (tqdm bar, default on as in the reference) and
+``conv_tol`` (> 0: stop once the GP energy changes by at most this relative
+amount per full step, checked every ``block_steps`` steps; the result then
+holds the steps actually run).
 """
 
 import numpy as np
@@ -20,6 +23,8 @@ from spinor_gpe.engine.plan import MAGIC_GAMMA, InstanceSpec, Plan
 from spinor_gpe.pspinor import prop_result
 from spinor_gpe.pspinor import tensor_tools as ttools
 from spinor_gpe.pspinor.plotting_tools import next_available_path
+
+ENERGY_KEYS = ('E_tot', 'E_kin', 'E_pot', 'E_int', 'E_cpl', 'mu', 'N', 'rel_change')
 
 try:
     from tqdm import tqdm as _tqdm
@@ -33,7 +38,7 @@ class TensorPropagator:
     # pylint: disable=too-many-instance-attributes,too-many-arguments
     def __init__(self, spin, t_step, n_steps, device='cpu', time='imag',
                  is_sampling=False, n_samples=1, *, precision='c128',
-                 block_steps=64, progress=True):
+                 block_steps=64, progress=True, conv_tol=0.0):
         """Set up a propagation (tensor_propagator.py:67-149)."""
         self._dev = nat.require_device(device)
         self.n_steps = n_steps
@@ -81,6 +86,7 @@ class TensorPropagator:
         self.precision = precision
         self.block_steps = int(block_steps)
         self.progress = progress
+        self.conv_tol = float(conv_tol)
         spec = InstanceSpec.from_spin(spin, time, t_step)
         self._plan = Plan(self._dev, precision, spec.ny, spec.nx, 1)
         self._state = self._plan.new_state(spin.psik)
@@ -130,14 +136,25 @@ class TensorPropagator:
         self._pending = True
 
     def _advance(self, n, bar):
+        """Run up to ``n`` full steps; returns the number run (fewer only if
+        the convergence test stopped the run)."""
         done = 0
+        chunk = self.block_steps if self.conv_tol > 0 else self.block_steps * 16
         while done < n:
-            k = min(self.block_steps * 16, n - done)
+            k = min(chunk, n - done)
             self._plan.run(k, block=self.block_steps)
             self._pending = True
             done += k
             if bar is not None:
                 bar.update(k)
+            if self.conv_tol > 0:
+                self._plan.energy()
+                self._pending = False
+                state, _ = self._plan.status()
+                if state[0, 1]:
+                    self._converged = True
+                    break
+        return done
 
     def prop_loop(self, n_steps):
         """Run ``n_steps`` full steps and return a :class:`PropResult`.
@@ -150,7 +167,8 @@ class TensorPropagator:
         pop_times = np.linspace(0, self.n_steps * np.abs(self.t_step), n_steps)
         pops = {'times': pop_times, 'vals': np.empty((n_steps, 2))}
         pops_dev = torch.zeros((max(n_steps, 1), 2), dtype=torch.float64, device=self._dev)
-        self._plan.set_run(0, pops_dev)
+        self._plan.set_run(0, pops_dev, conv_tol=self.conv_tol)
+        self._converged = False
 
         sample_at = []
         if self.is_sampling:
@@ -163,15 +181,24 @@ class TensorPropagator:
         bar = _tqdm(total=n_steps) if (self.progress and _tqdm is not None) else None
         done = 0
         for i in sample_at:
-            self._advance(i - done, bar)
-            done = i
+            if self._converged:
+                break
+            done += self._advance(i - done, bar)
+            if self._converged:
+                break
             idx = int(i / self.sample_rate)
             sampled_psik[idx] = np.array(ttools.to_numpy(self.psik))
-        self._advance(n_steps - done, bar)
+        if not self._converged:
+            done += self._advance(n_steps - done, bar)
         if bar is not None:
             bar.close()
         self._settle()
-        pops['vals'][:] = pops_dev[:n_steps].cpu().numpy()
+        if done < n_steps:  # stopped by the convergence test
+            pops['times'] = pop_times[:done]
+            pops['vals'] = np.empty((done, 2))
+        pops['vals'][:] = pops_dev[:done].cpu().numpy()
+        self._plan.set_active(0, True)
+        egp = self._plan.energy()[0].cpu().numpy()
 
         psik = ttools.to_numpy(self.psik)
         psi = ttools.to_numpy(ttools.ifft_2d(self.psik, self.space['dr']))
@@ -185,7 +212,10 @@ class TensorPropagator:
             file_name = None
 
         result = prop_result.PropResult(psi, psik, energy, pops, file_name)
-        result.steps_run = n_steps
+        result.steps_run = done
+        result.converged = self._converged
+        result.energy_gp = dict(zip(ENERGY_KEYS, (float(v) for v in egp)))
+        result.chem_pot = result.energy_gp['mu']
         return result
 
     # ------------------------------------------------------------ energy

@@ -48,6 +48,21 @@ def test_prop_loop_matches_oracle(gpu, case, precision):
     assert res.time_scale == ps.time_scale and res.paths is ps.paths
     np.testing.assert_allclose(res.psi[0], oe.ifft_2d_np(res.psik, ps.space['dr'])[0],
                                atol=1e-12 * np.abs(res.psi[0]).max() if precision == 'c128' else 1e-4)
+    # the engine's GP energy / chemical potential: the device evaluation vs
+    # the oracle's functional on the engine's own final state, and vs the
+    # oracle's final state
+    e_dev = res.energy_gp
+    cpl = ps.coupling if ps.is_coupling else None
+    theta = 2 * ps.kL_recoil * ps.space['x_mesh'] if (ps.is_coupling and not ps.rot_coupling) else None
+    e_own = oe.gp_energy(res.psik, ps.space['dr'], ps.kin_eng_spin, ps.pot_eng_spin, ps.g_sc, cpl, theta)
+    e_ref = oe.gp_energy(oracle.psik_numpy(), ps.space['dr'], ps.kin_eng_spin, ps.pot_eng_spin, ps.g_sc,
+                         cpl, theta)
+    tol_e = 1e-12 if precision == 'c128' else 1e-5
+    for key in ('E_tot', 'mu'):
+        assert e_dev[key] == pytest.approx(e_own[key], rel=tol_e), key
+        assert e_dev[key] == pytest.approx(e_ref[key], rel=max(tol_e, 10 * tol_psi)), key
+    assert e_dev['N'] == pytest.approx(ps.atom_num, rel=1e-12)
+    assert res.chem_pot == e_dev['mu']
     if precision == 'c128' and case in ('ground', 'raman_rot', 'tall'):
         # wrap-free states: the end-of-run energy is pinned against the oracle's
         want = oe.eng_expect(oracle.psik_numpy(), ps.space['dr'], ps.pot_eng_spin, ps.coupling,
@@ -114,6 +129,18 @@ def test_graph_and_eager_are_bit_identical(gpu):
         plan.sync()
         outs.append(st.cpu().numpy())
     np.testing.assert_array_equal(outs[0], outs[1])
+
+
+def test_convergence_stop(gpu):
+    """conv_tol stops the run once |dE/E| per step <= tol (checked per block)."""
+    ps, time, dt = CASES['ground'](name='conv')
+    res, prop = ps.imaginary(dt, 3000, 'cuda', progress=False, block_steps=50, conv_tol=1e-9)
+    assert res.converged and res.steps_run < 3000 and res.steps_run % 50 == 0
+    assert res.pops['vals'].shape == (res.steps_run, 2)
+    assert 0 <= res.energy_gp['rel_change'] or res.energy_gp['rel_change'] == -1
+    # one more block barely changes the energy
+    res2, _ = ps.imaginary(dt, 50, 'cuda', progress=False, block_steps=50)
+    assert abs(res2.energy_gp['E_tot'] / res.energy_gp['E_tot'] - 1) < 50 * 1e-9 * 2
 
 
 def test_cpu_device_raises(gpu):
