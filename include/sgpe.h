@@ -45,6 +45,7 @@ enum { SGPE_SUB_OUTER = 0, SGPE_SUB_INNER = 1 };  /* tensor_propagator.py:101-10
 
 typedef struct sgpe_ctx sgpe_ctx;
 typedef struct sgpe_plan sgpe_plan;
+typedef struct sgpe_group sgpe_group;
 
 /*
  * Physical description of one instance: the inputs TensorPropagator.__init__
@@ -190,6 +191,20 @@ int sgpe_plan_sync(sgpe_plan* plan);
  * spectral evaluation, used for convergence and chemical potential. Brings
  * the state to normalized k-space first (as sgpe_plan_sync). */
 int sgpe_energy(sgpe_plan* plan, double* out);
+
+/* ---------------------------------------------------- batched sweeps */
+
+/* A group of plans (e.g. one per grid size of a parameter sweep) advanced
+ * together. No counterpart in the reference, which propagates one instance
+ * per TensorPropagator (tensor_propagator.py:185-194); this batches many. */
+int sgpe_group_create(sgpe_ctx* ctx, int n_plans, sgpe_plan* const* plans, sgpe_group** out);
+void sgpe_group_destroy(sgpe_group* group);
+
+/* n_graphs launches of one captured hipGraph in which every plan, on its own
+ * concurrent branch, runs k full steps and, if check != 0, the energy
+ * evaluation and convergence test of sgpe_energy (results kept in the run
+ * state, see sgpe_plan_status). */
+int sgpe_group_run(sgpe_group* group, int k, int check, int n_graphs);
 
 /* ------------------------------------------------------------- host tools */
 

@@ -257,6 +257,41 @@ static void issue_sync(sgpe_plan* p, hipStream_t s) {
   p->last_dur = DUR_OUT;
 }
 
+struct sgpe_group {
+  sgpe_ctx* ctx = nullptr;
+  std::vector<sgpe_plan*> plans;
+  std::vector<hipStream_t> side;  // one capture branch per plan
+  std::vector<hipEvent_t> join;
+  hipEvent_t fork = nullptr;
+  std::map<long, hipGraphExec_t> graphs;
+  std::map<long, hipGraph_t> defs;
+};
+
+// Energy work buffer and kernel shapes of a plan (allocates once; must not
+// run during graph capture).
+static int plan_energy_prepare(sgpe_plan* p) {
+  if (p->scratch) return SGPE_OK;
+  hipStream_t s = p->ctx->stream;
+  const size_t cs = csize(p->prec);
+  const size_t per = (size_t)2 * p->ny * p->nx * cs;
+  int rc;
+  default_col(p->prec, p->ny, &p->e_col_e, &p->e_col_w);
+  default_row(p->prec, p->nx, &p->e_row_e, &p->e_row_rows);
+  p->ecol = find_entry(K_ECOL, p->prec, p->ny, p->e_col_e, p->e_col_w);
+  p->erow = find_entry(K_EROW, p->prec, p->nx, p->e_row_e, p->e_row_rows);
+  if (!p->ecol || !p->erow) return fail(SGPE_E_UNSUPPORTED, "energy kernels not compiled for this grid");
+  if ((rc = set_lds_attr(p->ecol->kernel, (size_t)p->e_col_w * p->ny * cs))) return rc;
+  if ((rc = set_lds_attr(p->erow->kernel, (size_t)p->e_row_rows * 2 * p->nx * cs))) return rc;
+  if (hipMalloc(&p->scratch, per * p->nslots) != hipSuccess) {
+    p->scratch = nullptr;
+    return fail(SGPE_E_NOMEM, "energy work buffer allocation failed");
+  }
+  for (int i = 0; i < p->nslots; ++i) p->h_inst[i].scratch = (char*)p->scratch + i * per;
+  HIP_TRY(hipMemcpyAsync(p->d_inst, p->h_inst.data(), p->nslots * sizeof(InstDev), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return SGPE_OK;
+}
+
 // Normalized k-space state, then the energy passes and the per-slot
 // reduction / convergence test (all stream-ordered kernels, graph-safe).
 static void issue_energy(sgpe_plan* p, hipStream_t s, double* out) {
@@ -873,29 +908,98 @@ int sgpe_energy(sgpe_plan* p, double* out) {
   int rc = plan_ready(p);
   if (rc) return rc;
   HIP_TRY(hipSetDevice(p->ctx->device));
-  hipStream_t s = p->ctx->stream;
-  const size_t cs = csize(p->prec);
-  if (!p->scratch) {
-    // Lazily allocated work state, one per slot; every slot's descriptor is
-    // re-uploaded with its pointer.
-    const size_t per = (size_t)2 * p->ny * p->nx * cs;
-    if (hipMalloc(&p->scratch, per * p->nslots) != hipSuccess) {
-      p->scratch = nullptr;
-      return fail(SGPE_E_NOMEM, "energy work buffer allocation failed");
-    }
-    for (int i = 0; i < p->nslots; ++i) p->h_inst[i].scratch = (char*)p->scratch + i * per;
-    HIP_TRY(hipMemcpyAsync(p->d_inst, p->h_inst.data(), p->nslots * sizeof(InstDev), hipMemcpyHostToDevice, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    default_col(p->prec, p->ny, &p->e_col_e, &p->e_col_w);
-    default_row(p->prec, p->nx, &p->e_row_e, &p->e_row_rows);
-    p->ecol = find_entry(K_ECOL, p->prec, p->ny, p->e_col_e, p->e_col_w);
-    p->erow = find_entry(K_EROW, p->prec, p->nx, p->e_row_e, p->e_row_rows);
-    if (!p->ecol || !p->erow) return fail(SGPE_E_UNSUPPORTED, "energy kernels not compiled for this grid");
-    if ((rc = set_lds_attr(p->ecol->kernel, (size_t)p->e_col_w * p->ny * cs))) return rc;
-    if ((rc = set_lds_attr(p->erow->kernel, (size_t)p->e_row_rows * 2 * p->nx * cs))) return rc;
-  }
-  issue_energy(p, s, out);
+  if ((rc = plan_energy_prepare(p))) return rc;
+  issue_energy(p, p->ctx->stream, out);
   HIP_TRY(hipGetLastError());
+  return SGPE_OK;
+}
+
+int sgpe_group_create(sgpe_ctx* ctx, int n_plans, sgpe_plan* const* plans, sgpe_group** out) {
+  if (!ctx || !out || n_plans <= 0 || !plans) return fail(SGPE_E_ARG, "bad arguments");
+  *out = nullptr;
+  for (int i = 0; i < n_plans; ++i) {
+    if (!plans[i] || plans[i]->ctx != ctx) return fail(SGPE_E_ARG, "plans must be non-NULL and share the context");
+    for (int j = 0; j < i; ++j)
+      if (plans[j] == plans[i]) return fail(SGPE_E_ARG, "a plan appears twice");
+  }
+  HIP_TRY(hipSetDevice(ctx->device));
+  sgpe_group* g = new sgpe_group();
+  g->ctx = ctx;
+  g->plans.assign(plans, plans + n_plans);
+  bool ok = hipEventCreateWithFlags(&g->fork, hipEventDisableTiming) == hipSuccess;
+  for (int i = 0; i < n_plans && ok; ++i) {
+    hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr;
+    ok = hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess;
+    if (ok) g->side.push_back(st);
+    ok = ok && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
+    if (ok) g->join.push_back(ev);
+  }
+  if (!ok) {
+    sgpe_group_destroy(g);
+    return fail(SGPE_E_HIP, "group stream/event creation failed");
+  }
+  *out = g;
+  return SGPE_OK;
+}
+
+void sgpe_group_destroy(sgpe_group* g) {
+  if (!g) return;
+  (void)hipSetDevice(g->ctx->device);
+  (void)hipStreamSynchronize(g->ctx->stream);
+  for (auto& e : g->graphs) (void)hipGraphExecDestroy(e.second);
+  for (auto& e : g->defs) (void)hipGraphDestroy(e.second);
+  for (auto s : g->side) (void)hipStreamDestroy(s);
+  for (auto e : g->join) (void)hipEventDestroy(e);
+  if (g->fork) (void)hipEventDestroy(g->fork);
+  delete g;
+}
+
+int sgpe_group_run(sgpe_group* g, int k, int check, int n_graphs) {
+  if (!g || k <= 0 || n_graphs < 0) return fail(SGPE_E_ARG, "bad arguments");
+  HIP_TRY(hipSetDevice(g->ctx->device));
+  int rc;
+  for (auto* p : g->plans) {
+    if ((rc = plan_ready(p))) return rc;
+    if (check && (rc = plan_energy_prepare(p))) return rc;
+    if (p->last_dur != DUR_OUT) issue_sync(p, g->ctx->stream);
+  }
+  if (n_graphs == 0) return SGPE_OK;
+  const long key = 2L * k + (check ? 1 : 0);
+  auto it = g->graphs.find(key);
+  if (it == g->graphs.end()) {
+    // One capture: the context stream forks into one branch per plan, each
+    // branch runs k full steps (and the energy/convergence check), and the
+    // branches join back, so different grid sizes run concurrently.
+    hipStream_t cs = g->ctx->cap_stream;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipEventRecord(g->fork, cs);
+    for (size_t i = 0; i < g->plans.size() && e == hipSuccess; ++i) {
+      e = hipStreamWaitEvent(g->side[i], g->fork, 0);
+      if (e != hipSuccess) break;
+      for (int j = 0; j < k; ++j) issue_full_step(g->plans[i], g->side[i]);
+      if (check) issue_energy(g->plans[i], g->side[i], nullptr);
+      e = hipEventRecord(g->join[i], g->side[i]);
+      if (e == hipSuccess) e = hipStreamWaitEvent(cs, g->join[i], 0);
+    }
+    hipError_t le = hipGetLastError();
+    hipError_t ce = hipStreamEndCapture(cs, &graph);
+    if (e != hipSuccess || le != hipSuccess || ce != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      hipError_t bad = e != hipSuccess ? e : (le != hipSuccess ? le : ce);
+      return fail(SGPE_E_HIP, std::string("group graph capture failed: ") + hipGetErrorString(bad));
+    }
+    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      return fail(SGPE_E_HIP, "group graph instantiation failed");
+    }
+    g->defs[key] = graph;
+    it = g->graphs.emplace(key, exec).first;
+  }
+  for (int i = 0; i < n_graphs; ++i) HIP_TRY(hipGraphLaunch(it->second, g->ctx->stream));
+  for (auto* p : g->plans) p->last_dur = DUR_OUT;
   return SGPE_OK;
 }
 

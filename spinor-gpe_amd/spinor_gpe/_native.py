@@ -83,6 +83,9 @@ def _declare(lib):
         'sgpe_plan_sync': (i, [vp]),
         'sgpe_profile_full_steps': (i, [vp, i, _dp]),
         'sgpe_energy': (i, [vp, vp]),
+        'sgpe_group_create': (i, [vp, i, ctypes.POINTER(vp), ctypes.POINTER(vp)]),
+        'sgpe_group_destroy': (None, [vp]),
+        'sgpe_group_run': (i, [vp, i, i, i]),
         'sgpe_unwrap_phase_2d': (i, [_dp, _dp, i, i]),
     }
     for name, (res, args) in sig.items():

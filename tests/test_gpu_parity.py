@@ -61,7 +61,7 @@ def test_prop_loop_matches_oracle(gpu, case, precision):
     for key in ('E_tot', 'mu'):
         assert e_dev[key] == pytest.approx(e_own[key], rel=tol_e), key
         assert e_dev[key] == pytest.approx(e_ref[key], rel=max(tol_e, 10 * tol_psi)), key
-    assert e_dev['N'] == pytest.approx(ps.atom_num, rel=1e-12)
+    assert e_dev['N'] == pytest.approx(ps.atom_num, rel=1e-12 if precision == 'c128' else 1e-6)
     assert res.chem_pot == e_dev['mu']
     if precision == 'c128' and case in ('ground', 'raman_rot', 'tall'):
         # wrap-free states: the end-of-run energy is pinned against the oracle's

@@ -1,0 +1,65 @@
+"""Batched sweeps: many instances of mixed grid sizes in one graph-driven loop
+(GPU). Each instance's result must equal its solve alone (same precision)
+and, in complex64, agree with the complex128 solve to the stated tolerance."""
+
+import numpy as np
+import pytest
+
+from cases import base, kl3_scale, WAVEL
+from spinor_gpe.engine.plan import InstanceSpec
+from spinor_gpe.engine.sweep import Sweep
+
+pytestmark = pytest.mark.gpu
+
+
+def instances(n, seed=11):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        mesh = [(64, 64), (128, 128), (64, 128)][i % 3]
+        ps = base(f'sw{i}', mesh=mesh, r=(16, 16) if mesh[0] == mesh[1] else (16, 32),
+                  atoms=10 ** rng.uniform(3, 4), g=(1.0, 1.0, rng.uniform(0.9, 1.1)))
+        ps.coupling_setup(wavel=WAVEL, scale=kl3_scale(ps), kin_shift=True)
+        ps.coupling_uniform(rng.uniform(0, 4) * ps.EL_recoil)
+        ps.detuning_uniform(rng.uniform(-1, 1) * ps.EL_recoil)
+        out.append(ps)
+    return out
+
+
+@pytest.mark.parametrize('precision', ['c128', 'c64'])
+def test_sweep_alone_vs_batch(gpu, precision):
+    spins = instances(6)
+    batch = Sweep(gpu, precision, conv_tol=1e-7, check_every=25, max_steps=1500)
+    for ps in spins:
+        batch.add_spin(ps, 0.02)
+    res, wall = batch.run()
+    assert wall > 0
+    for i, ps in enumerate(spins):
+        print(f"[sweep] {precision} instance {i} grid {res[i].grid} steps {res[i].steps} "
+              f"converged {res[i].converged} E {res[i].energy['E_tot']:.12g}", flush=True)
+        alone = Sweep(gpu, precision, conv_tol=1e-7, check_every=25, max_steps=1500)
+        alone.add_spin(ps, 0.02)
+        (one,), _ = alone.run()
+        assert one.steps == res[i].steps and one.converged == res[i].converged
+        # identical kernels and reduction order: the batch changes nothing
+        np.testing.assert_array_equal(one.psik, res[i].psik)
+        assert one.energy == res[i].energy
+        assert res[i].energy['N'] == pytest.approx(ps.atom_num, rel=1e-6)
+
+
+def test_sweep_c64_matches_c128(gpu):
+    spins = instances(4, seed=5)
+    out = {}
+    for prec in ('c128', 'c64'):
+        sw = Sweep(gpu, prec, conv_tol=1e-8, check_every=50, max_steps=8000)
+        for ps in spins:
+            sw.add_spin(ps, 0.02)
+        out[prec], _ = sw.run()
+    for a, b in zip(out['c128'], out['c64']):
+        assert a.converged and b.converged
+        assert abs(b.energy['E_tot'] / a.energy['E_tot'] - 1) < 1e-6
+        fid = abs(np.vdot(a.psik, b.psik)) ** 2 / (np.vdot(a.psik, a.psik).real * np.vdot(b.psik, b.psik).real)
+        print(f"[sweep c64 vs c128] grid {a.grid} steps {a.steps}/{b.steps} "
+              f"dE/E {b.energy['E_tot'] / a.energy['E_tot'] - 1:.2e} infidelity {1 - fid:.2e}", flush=True)
+        # the two runs may stop a check interval apart on a slow mode
+        assert fid > 1 - 1e-5
