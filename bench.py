@@ -89,8 +89,13 @@ def c1_spinor(device):
     return ps
 
 
-def time_engine(ps, device, precision, steps, warmup, block, prof_steps):
+def time_engine(ps, device, precision, steps, warmup, block, prof_steps, tune):
+    from spinor_gpe.engine import autotune
     from spinor_gpe.pspinor.tensor_propagator import TensorPropagator
+    if tune:
+        shape, timings = autotune.tune(torch.device(device), precision, 512, 512, 1)
+        autotune.store(precision, 512, 512, 1, shape, timings)
+        log(f"tuned {precision}: {shape} {timings}")
     prop = TensorPropagator(ps, 1e-3, steps, device, time='real', precision=precision,
                             block_steps=block, progress=False)
     plan = prop._plan  # pylint: disable=protected-access
@@ -101,7 +106,9 @@ def time_engine(ps, device, precision, steps, warmup, block, prof_steps):
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     col_ms, row_ms, n_col, n_row = plan.profile_steps(prof_steps)
-    atoms = sum(float(torch.sum(torch.abs(p) ** 2).item()) for p in prop.psik) * ps.space['dv_k']
+    plan.sync()
+    st = plan.states[0]
+    atoms = float(torch.sum(torch.abs(st) ** 2).item()) * ps.space['dv_k']
     return {'elapsed': elapsed, 'col_ms': col_ms / n_col, 'row_ms': row_ms / n_row,
             'tuning': plan.tuning(), 'atoms': atoms}
 
@@ -133,24 +140,37 @@ def main():
     ap.add_argument('--profile-steps', type=int, default=200)
     ap.add_argument('--cpu-budget', type=float, default=15.0, help='seconds of CPU baseline sampling')
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-tune', action='store_true', help='use cached/default FFT shapes')
+    ap.add_argument('--precisions', nargs='+', default=['c128', 'c64'], choices=['c128', 'c64'])
+    ap.add_argument('--calibrate-copy', type=int, default=0, metavar='MIB',
+                    help='before timing, copy a MIB-sized device buffer once (a known-size '
+                         'copy for scaling HBM counters in profiles)')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine measures one GPU; --gpus {args.gpus} is not supported "
-              "(instances shard without communication; see DESIGN.md)", file=sys.stderr)
+              "(instances shard across GPUs without communication)", file=sys.stderr)
         sys.exit(2)
     if not torch.cuda.is_available():
         print("bench.py: no HIP device available", file=sys.stderr)
         sys.exit(1)
     os.environ.setdefault('SPINOR_GPE_DATA', tempfile.mkdtemp(prefix='sgpe_bench_'))
+    os.environ.setdefault('SGPE_TUNE_FILE', os.path.join(tempfile.mkdtemp(prefix='sgpe_tune_'), 'tune.json'))
     device = 'cuda:0'
+    if args.calibrate_copy:
+        src = torch.ones(args.calibrate_copy * 2 ** 20 // 4, dtype=torch.float32, device=device)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calibrate_copy} MiB read + {args.calibrate_copy} MiB written")
+        del src, dst
     log("building the 512^2 initial state (256^2 ground state, 4000 imaginary steps)")
     ps = c1_spinor(device)
     n_pts = 512 * 512
     results = {}
-    for prec in ('c128', 'c64'):
+    for prec in args.precisions:
         log(f"timing {prec}: warmup {args.warmup}, steps {args.steps}")
         results[prec] = time_engine(ps, device, prec, args.steps, args.warmup, args.block,
-                                    args.profile_steps)
+                                    args.profile_steps, not args.no_tune)
     r = results['c128']
     sps = args.steps / r['elapsed']
     bytes_c = {'c128': 16, 'c64': 8}
@@ -165,8 +185,7 @@ def main():
                 'traffic': None, 'avg_us': round(ms * 1e3, 3), 'algorithmic_bytes': alg,
                 'col_pass_us': round(res['col_ms'] * 1e3, 3), 'row_pass_us': round(res['row_ms'] * 1e3, 3)}
 
-    r64 = results['c64']
-    sps64 = args.steps / r64['elapsed']
+    r64 = results.get('c64')
     line = {
         'metric': METRIC, 'value': round(sps, 2), 'unit': 'split-steps/s', 'n_gpus': 1,
         'steps': args.steps, 'warmup': args.warmup,
@@ -182,11 +201,13 @@ def main():
                                '(reference bench_data/TitanV_cuda_step.npz)'},
         'grid_point_steps_per_s': round(sps * n_pts, 1),
         'roofline': roofline(r, 'c128'),
-        'c64': {'value': round(sps64, 2), 'ms_per_step': round(r64['elapsed'] / args.steps * 1e3, 5),
-                'grid_point_steps_per_s': round(sps64 * n_pts, 1), 'roofline': roofline(r64, 'c64'),
-                'dtype': 'c64 (f32 arithmetic, f64 reductions)'},
-        'atoms_after_run': {'c128': r['atoms'], 'c64': r64['atoms']},
+        'atoms_after_run': {p: res['atoms'] for p, res in results.items()},
     }
+    if r64:
+        sps64 = args.steps / r64['elapsed']
+        line['c64'] = {'value': round(sps64, 2), 'ms_per_step': round(r64['elapsed'] / args.steps * 1e3, 5),
+                       'grid_point_steps_per_s': round(sps64 * n_pts, 1), 'roofline': roofline(r64, 'c64'),
+                       'dtype': 'c64 (f32 arithmetic, f64 reductions)'}
     if not args.no_cpu_baseline:
         log(f"CPU baseline: oracle full_step() for ~{args.cpu_budget:.0f} s")
         line['cpu_baseline'] = cpu_baseline(ps, args.cpu_budget)

@@ -153,6 +153,11 @@ int sgpe_plan_set_active(sgpe_plan* plan, int slot, int active);
  * 0 keeps the current value. Returns SGPE_E_UNSUPPORTED if not compiled. */
 int sgpe_plan_tune(sgpe_plan* plan, int row_e, int row_rows, int col_e, int col_w);
 
+/* Compiled shapes for one pass (0 = rows: {E, rows per block}, 1 = columns:
+ * {E, columns per block}) of length n; writes up to cap pairs and returns
+ * the number available (>= 0) or a negative error. Feeds the autotuner. */
+int sgpe_list_shapes(int pass, int precision, int n, int* out_pairs, int cap);
+
 /* Current shapes: out4 = {row_e, row_rows, col_e, col_w}. */
 int sgpe_plan_get_tuning(const sgpe_plan* plan, int* out4);
 

@@ -777,6 +777,23 @@ int sgpe_plan_set_active(sgpe_plan* p, int slot, int active) {
   return SGPE_OK;
 }
 
+int sgpe_list_shapes(int pass, int precision, int n, int* out_pairs, int cap) {
+  if ((pass != 0 && pass != 1) || (precision != SGPE_C128 && precision != SGPE_C64) || cap < 0)
+    return fail(SGPE_E_ARG, "bad arguments");
+  const KTable t = precision == SGPE_C128 ? registry_f64(pass == 0 ? K_ROW : K_COL)
+                                          : registry_f32(pass == 0 ? K_ROW : K_COL);
+  int count = 0;
+  for (int i = 0; i < t.count; ++i) {
+    if (t.entries[i].n != n) continue;
+    if (out_pairs && count < cap) {
+      out_pairs[2 * count] = t.entries[i].e;
+      out_pairs[2 * count + 1] = t.entries[i].p;
+    }
+    ++count;
+  }
+  return count;
+}
+
 int sgpe_plan_tune(sgpe_plan* p, int row_e, int row_rows, int col_e, int col_w) {
   if (!p) return fail(SGPE_E_ARG, "plan is NULL");
   HIP_TRY(hipSetDevice(p->ctx->device));

@@ -24,9 +24,79 @@ namespace sgpe {
 
 // ---------------------------------------------------------------- helpers
 
+// sin and cos by Cody-Waite reduction x = k pi/2 + r, |r| <= pi/4, with
+// pi/2 split into three parts (the first two with 33 and 30 significant bits,
+// so k * part is exact for |k| < 2^20), then Taylor polynomials in r (terms
+// through r^17 / r^18 for double, r^9 / r^10 for float: truncation below half
+// an ulp). Within 2 ulp of the correctly rounded values for |x| <= 1e5; larger
+// arguments use the device library.
+__device__ __forceinline__ double cw_reduce(double x, int* quadrant) {
+  const double k = rint(x * 0.6366197723675814);
+  double r = fma(-k, 1.5707963267341256, x);
+  r = fma(-k, 6.077100506303966e-11, r);
+  r = fma(-k, 2.0222662487959506e-21, r);
+  *quadrant = (int)k & 3;
+  return r;
+}
+
+template <typename R> __device__ __forceinline__ void quadrant_fix(int q, R sr, R cr, R* s, R* c) {
+  const R ss = (q & 1) ? cr : sr;
+  const R cc = (q & 1) ? sr : cr;
+  *s = (q & 2) ? -ss : ss;
+  *c = ((q + 1) & 2) ? -cc : cc;
+}
+
 template <typename R> __device__ __forceinline__ void sc(R x, R* s, R* c);
-template <> __device__ __forceinline__ void sc<double>(double x, double* s, double* c) { sincos(x, s, c); }
-template <> __device__ __forceinline__ void sc<float>(float x, float* s, float* c) { sincosf(x, s, c); }
+template <> __device__ __forceinline__ void sc<double>(double x, double* s, double* c) {
+  if (!(fabs(x) <= 1.0e5)) {
+    sincos(x, s, c);
+    return;
+  }
+  int q;
+  const double r = cw_reduce(x, &q);
+  const double r2 = r * r;
+  double ps = 2.8114572543455206e-15;
+  ps = fma(ps, r2, -7.647163731819816e-13);
+  ps = fma(ps, r2, 1.6059043836821613e-10);
+  ps = fma(ps, r2, -2.505210838544172e-08);
+  ps = fma(ps, r2, 2.7557319223985893e-06);
+  ps = fma(ps, r2, -0.0001984126984126984);
+  ps = fma(ps, r2, 0.008333333333333333);
+  ps = fma(ps, r2, -0.16666666666666666);
+  const double sr = fma(r * r2, ps, r);
+  double pc = -1.5619206968586225e-16;
+  pc = fma(pc, r2, 4.779477332387385e-14);
+  pc = fma(pc, r2, -1.1470745597729725e-11);
+  pc = fma(pc, r2, 2.08767569878681e-09);
+  pc = fma(pc, r2, -2.755731922398589e-07);
+  pc = fma(pc, r2, 2.48015873015873e-05);
+  pc = fma(pc, r2, -0.001388888888888889);
+  pc = fma(pc, r2, 0.041666666666666664);
+  pc = fma(pc, r2, -0.5);
+  const double cr = fma(r2, pc, 1.0);
+  quadrant_fix<double>(q, sr, cr, s, c);
+}
+template <> __device__ __forceinline__ void sc<float>(float x, float* s, float* c) {
+  if (!(fabsf(x) <= 1.0e5f)) {
+    sincosf(x, s, c);
+    return;
+  }
+  int q;
+  const float r = (float)cw_reduce((double)x, &q);
+  const float r2 = r * r;
+  float ps = 2.7557319223985893e-06f;
+  ps = fmaf(ps, r2, -0.0001984126984126984f);
+  ps = fmaf(ps, r2, 0.008333333333333333f);
+  ps = fmaf(ps, r2, -0.16666666666666666f);
+  const float sr = fmaf(r * r2, ps, r);
+  float pc = -2.755731922398589e-07f;
+  pc = fmaf(pc, r2, 2.48015873015873e-05f);
+  pc = fmaf(pc, r2, -0.001388888888888889f);
+  pc = fmaf(pc, r2, 0.041666666666666664f);
+  pc = fmaf(pc, r2, -0.5f);
+  const float cr = fmaf(r2, pc, 1.0f);
+  quadrant_fix<float>(q, sr, cr, s, c);
+}
 template <typename R> __device__ __forceinline__ R ex(R x);
 template <> __device__ __forceinline__ double ex<double>(double x) { return exp(x); }
 template <> __device__ __forceinline__ float ex<float>(float x) { return expf(x); }

@@ -77,17 +77,24 @@ void erow_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const InstDev* i, cons
   {PR, N, E, RW, (void*)&sgpe::erow_launch<R, N, E, RW>, (const void*)&sgpe::energy_row_kernel<R, N, E, RW>},
 
 // Tunable shapes of the two split-step passes: columns (N, E, W), rows (N, E, ROWS).
-#define SGPE_COL_SHAPES(X, R, PR)                                                                         \
-  X(R, PR, 64, 8, 8) X(R, PR, 64, 8, 16) X(R, PR, 128, 8, 4) X(R, PR, 128, 8, 8) X(R, PR, 128, 8, 16)      \
-  X(R, PR, 256, 8, 4) X(R, PR, 256, 8, 8) X(R, PR, 256, 16, 8) X(R, PR, 512, 8, 2) X(R, PR, 512, 8, 4)    \
-  X(R, PR, 512, 8, 8) X(R, PR, 512, 16, 4) X(R, PR, 512, 16, 8) X(R, PR, 1024, 8, 2) X(R, PR, 1024, 8, 4) \
-  X(R, PR, 1024, 16, 4) X(R, PR, 1024, 16, 8) X(R, PR, 2048, 8, 2) X(R, PR, 2048, 8, 4)                 \
-  X(R, PR, 2048, 16, 2) X(R, PR, 2048, 16, 4) X(R, PR, 4096, 16, 2) X(R, PR, 4096, 16, 4)
-#define SGPE_ROW_SHAPES(X, R, PR)                                                                           \
-  X(R, PR, 64, 8, 8) X(R, PR, 64, 8, 16) X(R, PR, 128, 8, 4) X(R, PR, 128, 8, 8) X(R, PR, 256, 8, 2)        \
-  X(R, PR, 256, 8, 4) X(R, PR, 256, 16, 4) X(R, PR, 512, 8, 1) X(R, PR, 512, 8, 2) X(R, PR, 512, 16, 2)     \
-  X(R, PR, 512, 16, 4) X(R, PR, 1024, 8, 1) X(R, PR, 1024, 16, 1) X(R, PR, 1024, 16, 2) X(R, PR, 2048, 8, 1) \
-  X(R, PR, 2048, 16, 1) X(R, PR, 4096, 16, 1)
+#define SGPE_COL_SHAPES(X, R, PR)                                                                          \
+  X(R, PR, 64, 2, 4) X(R, PR, 64, 4, 4) X(R, PR, 64, 4, 8) X(R, PR, 64, 8, 8) X(R, PR, 64, 8, 16)           \
+  X(R, PR, 128, 2, 4) X(R, PR, 128, 4, 4) X(R, PR, 128, 4, 8) X(R, PR, 128, 8, 4) X(R, PR, 128, 8, 8)       \
+  X(R, PR, 128, 8, 16) X(R, PR, 256, 2, 2) X(R, PR, 256, 2, 4) X(R, PR, 256, 4, 4) X(R, PR, 256, 4, 8)      \
+  X(R, PR, 256, 8, 4) X(R, PR, 256, 8, 8) X(R, PR, 256, 16, 8) X(R, PR, 512, 2, 2) X(R, PR, 512, 2, 4)      \
+  X(R, PR, 512, 4, 2) X(R, PR, 512, 4, 4) X(R, PR, 512, 8, 2) X(R, PR, 512, 8, 4) X(R, PR, 512, 8, 8)       \
+  X(R, PR, 512, 16, 4) X(R, PR, 512, 16, 8) X(R, PR, 1024, 4, 2) X(R, PR, 1024, 4, 4) X(R, PR, 1024, 8, 2)  \
+  X(R, PR, 1024, 8, 4) X(R, PR, 1024, 16, 4) X(R, PR, 1024, 16, 8) X(R, PR, 2048, 4, 2) X(R, PR, 2048, 8, 2) \
+  X(R, PR, 2048, 8, 4) X(R, PR, 2048, 16, 2) X(R, PR, 2048, 16, 4) X(R, PR, 4096, 8, 2) X(R, PR, 4096, 16, 2) \
+  X(R, PR, 4096, 16, 4)
+#define SGPE_ROW_SHAPES(X, R, PR)                                                                          \
+  X(R, PR, 64, 2, 2) X(R, PR, 64, 4, 4) X(R, PR, 64, 8, 8) X(R, PR, 64, 8, 16) X(R, PR, 128, 2, 1)          \
+  X(R, PR, 128, 2, 2) X(R, PR, 128, 4, 2) X(R, PR, 128, 4, 4) X(R, PR, 128, 8, 4) X(R, PR, 128, 8, 8)       \
+  X(R, PR, 256, 2, 1) X(R, PR, 256, 4, 1) X(R, PR, 256, 4, 2) X(R, PR, 256, 8, 2) X(R, PR, 256, 8, 4)       \
+  X(R, PR, 256, 16, 4) X(R, PR, 512, 2, 1) X(R, PR, 512, 4, 1) X(R, PR, 512, 4, 2) X(R, PR, 512, 8, 1)      \
+  X(R, PR, 512, 8, 2) X(R, PR, 512, 16, 2) X(R, PR, 512, 16, 4) X(R, PR, 1024, 4, 1) X(R, PR, 1024, 8, 1)   \
+  X(R, PR, 1024, 16, 1) X(R, PR, 1024, 16, 2) X(R, PR, 2048, 4, 1) X(R, PR, 2048, 8, 1) X(R, PR, 2048, 16, 1) \
+  X(R, PR, 4096, 8, 1) X(R, PR, 4096, 16, 1)
 
 // Default shapes (standalone FFTs and energy evaluation use only these);
 // they must match default_row / default_col in sgpe_api.hip.

@@ -21,8 +21,8 @@ C128, C64 = 0, 1
 TIME_IMAG, TIME_REAL = 0, 1
 COUPLING_OFF, COUPLING_UNIFORM, COUPLING_ARRAY = 0, 1, 2
 SUB_OUTER, SUB_INNER = 0, 1
-PRECISIONS = {'c128': C128, 'complex128': C128, torch.complex128: C128,
-              'c64': C64, 'complex64': C64, torch.complex64: C64}
+PRECISIONS = {'c128': C128, 'complex128': C128, torch.complex128: C128, C128: C128,
+              'c64': C64, 'complex64': C64, torch.complex64: C64, C64: C64}
 TORCH_DTYPE = {C128: torch.complex128, C64: torch.complex64}
 REAL_DTYPE = {C128: torch.float64, C64: torch.float32}
 
@@ -76,6 +76,7 @@ def _declare(lib):
         'sgpe_plan_set_active': (i, [vp, i, i]),
         'sgpe_plan_status': (i, [vp, ip, _dp]),
         'sgpe_plan_tune': (i, [vp, i, i, i, i]),
+        'sgpe_list_shapes': (i, [i, i, i, ip, i]),
         'sgpe_plan_get_tuning': (i, [vp, ip]),
         'sgpe_single_step': (i, [vp, i]),
         'sgpe_full_steps': (i, [vp, i]),

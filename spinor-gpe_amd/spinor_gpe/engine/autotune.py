@@ -1,0 +1,182 @@
+"""FFT-shape autotuner for the split-step passes.
+
+For a grid length the library compiles several shapes of each pass: E
+elements per thread (the FFT's radix) and the number of rows (row pass) or
+columns (column pass) per workgroup. Which one is fastest depends on the
+grid, the precision and how many instances share a launch (small single
+instances are latency-bound and want many small waves; large or batched ones
+want fewer, fatter threads). The tuner times every compiled shape with
+``sgpe_profile_full_steps`` on a scratch plan (the caller's states are never
+touched), picks the fastest row and column shapes independently and caches
+the choice in a JSON file keyed by precision, grid and batch-size bucket.
+
+Usage: ``python -m spinor_gpe.engine.autotune --grids 128 256 512 1024 2048``
+writes the cache (``$SGPE_TUNE_FILE``, default
+``~/.cache/spinor_gpe/tune.json``).
+"""
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+from spinor_gpe import _native as nat
+from spinor_gpe.engine.plan import InstanceSpec, Plan
+
+PACKAGED = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'tuned_mi355x.json')
+
+
+def cache_path():
+    return os.environ.get('SGPE_TUNE_FILE',
+                          os.path.join(os.path.expanduser('~'), '.cache', 'spinor_gpe', 'tune.json'))
+
+
+def batch_bucket(n_slots):
+    for b in (1, 4, 16, 64):
+        if n_slots <= b:
+            return b
+    return 256
+
+
+def key(precision, ny, nx, n_slots):
+    return f"{nat.precision_of(precision)}:{ny}x{nx}:{batch_bucket(n_slots)}"
+
+
+def _load(path):
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def lookup(precision, ny, nx, n_slots):
+    """Cached (row_e, row_rows, col_e, col_w) or None (user cache, then packaged)."""
+    k = key(precision, ny, nx, n_slots)
+    for path in (cache_path(), PACKAGED):
+        entry = _load(path).get(k)
+        if entry:
+            return tuple(entry['shape'])
+    return None
+
+
+def store(precision, ny, nx, n_slots, shape, timings):
+    path = cache_path()
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    data = _load(path)
+    data[key(precision, ny, nx, n_slots)] = {'shape': list(shape), 'timings_us': timings,
+                                             'when': time.strftime('%Y-%m-%d %H:%M:%S')}
+    tmp = path + '.tmp'
+    with open(tmp, 'w') as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+    os.replace(tmp, path)
+
+
+def shapes(pass_, precision, n):
+    """Compiled (E, rows-or-columns) pairs of one pass (0 rows, 1 columns)."""
+    prec = nat.precision_of(precision)
+    cnt = nat.lib().sgpe_list_shapes(pass_, prec, n, None, 0)
+    if cnt < 0:
+        nat.check(cnt)
+    buf = (ctypes.c_int * (2 * cnt))()
+    nat.lib().sgpe_list_shapes(pass_, prec, n, buf, cnt)
+    return [(buf[2 * i], buf[2 * i + 1]) for i in range(cnt)]
+
+
+def synthetic_spec(ny, nx, time_mode='real'):
+    """A Raman-coupled Thomas-Fermi-like instance of the given grid (lab frame)."""
+    r = 16.0
+    x = np.linspace(-r, r, nx, endpoint=False)
+    y = np.linspace(-r, r, ny, endpoint=False)
+    kx = np.linspace(-np.pi * nx / (2 * r), np.pi * nx / (2 * r), nx, endpoint=False)
+    ky = np.linspace(-np.pi * ny / (2 * r), np.pi * ny / (2 * r), ny, endpoint=False)
+    xm, ym = np.meshgrid(x, y)
+    kxm, kym = np.meshgrid(kx, ky)
+    pot = (xm ** 2 + ym ** 2) / 2
+    kin = (kxm ** 2 + kym ** 2) / 2
+    psi = np.sqrt(np.maximum(30.0 - pot, 0.0)) + 0j
+    c = (2 * r / nx) * (2 * r / ny) / (2 * np.pi)
+    psik = np.fft.fftshift(np.fft.fftn(psi)) * c
+    spec = InstanceSpec([kin, kin], [pot + 1.0, pot - 1.0], [2 * r / nx, 2 * r / ny], 1e4,
+                        {'uu': 0.11, 'dd': 0.11, 'ud': 0.105}, time_mode, 1e-3,
+                        coupling=np.full((ny, nx), 9.0), theta_x=6.0 * x)
+    return spec, np.stack([psik, psik])
+
+
+def tune(device, precision, ny, nx, n_slots=1, steps=30, reps=3, verbose=False):
+    """Time every compiled shape; returns (best shape, timings dict in us)."""
+    spec, psik = synthetic_spec(ny, nx)
+    plan = Plan(device, precision, ny, nx, n_slots)
+    states = torch.empty((n_slots, 2, ny, nx), dtype=plan.dtype, device=plan.device)
+    states.copy_(torch.as_tensor(np.broadcast_to(psik, (n_slots,) + psik.shape).copy(),
+                                 dtype=plan.dtype))
+    for s in range(n_slots):
+        plan.set_slot(s, spec, states[s])
+    base = plan.tuning()
+    timings = {'row': {}, 'col': {}}
+    best = list(base)
+    for pass_, name in ((0, 'row'), (1, 'col')):
+        best_t = float('inf')
+        for e, p in shapes(pass_, precision, nx if pass_ == 0 else ny):
+            try:
+                if pass_ == 0:
+                    plan.tune(row_e=e, row_rows=p, col_e=base[2], col_w=base[3])
+                else:
+                    plan.tune(row_e=base[0], row_rows=base[1], col_e=e, col_w=p)
+            except nat.SgpeError:
+                continue  # exceeds LDS or does not tile this grid
+            plan.profile_steps(2)
+            idx = 1 if pass_ == 0 else 0  # profile_steps -> (col ms, row ms, ...)
+            t = min(plan.profile_steps(steps)[idx] for _ in range(reps)) / (3 * steps) * 1e3
+            timings[name][f"{e}x{p}"] = round(t, 3)
+            if verbose:
+                print(f"  {name} E={e} {p}: {t:.2f} us", flush=True)
+            if t < best_t:
+                best_t = t
+                if pass_ == 0:
+                    best[0], best[1] = e, p
+                else:
+                    best[2], best[3] = e, p
+    plan.tune(*best)
+    return tuple(best), timings
+
+
+def apply(plan):
+    """Set a plan's shapes from the cache if an entry exists; returns it."""
+    shape = lookup(plan.prec, plan.ny, plan.nx, plan.n_slots)
+    if shape:
+        try:
+            plan.tune(*shape)
+        except nat.SgpeError:
+            return None
+    return shape
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description="Tune FFT shapes of the split-step passes.")
+    ap.add_argument('--grids', type=int, nargs='+', default=[128, 256, 512, 1024, 2048])
+    ap.add_argument('--precisions', nargs='+', default=['c128', 'c64'])
+    ap.add_argument('--slots', type=int, nargs='+', default=[1])
+    ap.add_argument('--steps', type=int, default=30)
+    ap.add_argument('--out', default=None, help='cache file (default: $SGPE_TUNE_FILE)')
+    args = ap.parse_args(argv)
+    if args.out:
+        os.environ['SGPE_TUNE_FILE'] = args.out
+    dev = torch.device('cuda', 0)
+    for prec in args.precisions:
+        for n in args.grids:
+            for slots in args.slots:
+                print(f"[tune] {prec} {n}x{n} slots={slots}", flush=True)
+                shape, t = tune(dev, prec, n, n, slots, steps=args.steps, verbose=True)
+                store(prec, n, n, slots, shape, t)
+                print(f"[tune] best {shape}", flush=True)
+    return 0
+
+
+if __name__ == '__main__':
+    sys.exit(main())

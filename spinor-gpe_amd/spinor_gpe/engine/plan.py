@@ -11,6 +11,7 @@ step. Anything else is passed densely.
 """
 
 import ctypes
+import os
 
 import numpy as np
 import torch
@@ -131,6 +132,9 @@ class Plan:
                                              self.n_slots, ctypes.byref(handle)))
         self.handle = handle
         self.states = [None] * self.n_slots
+        if not os.environ.get('SGPE_NO_TUNE_CACHE'):
+            from spinor_gpe.engine import autotune  # pylint: disable=import-outside-toplevel
+            autotune.apply(self)
 
     def __del__(self):
         h = getattr(self, 'handle', None)

@@ -115,3 +115,31 @@ def test_native_unwrap_ramp_and_identity():
     assert np.allclose(d, d[0, 0], atol=1e-12)
     small = 0.4 * np.cos(xx / 6.0) * np.sin(yy / 4.0)
     np.testing.assert_array_equal(nat.unwrap_phase(small), small)
+
+
+def test_precision_codes():
+    for v in ('c128', 'complex128', nat.C128, __import__('torch').complex128):
+        assert nat.precision_of(v) == nat.C128
+    for v in ('c64', 'complex64', nat.C64, __import__('torch').complex64):
+        assert nat.precision_of(v) == nat.C64
+    with pytest.raises(ValueError):
+        nat.precision_of('f16')
+
+
+def test_autotune_cache_roundtrip(tmp_path, monkeypatch):
+    from spinor_gpe.engine import autotune
+    monkeypatch.setenv('SGPE_TUNE_FILE', str(tmp_path / 'tune.json'))
+    assert autotune.batch_bucket(1) == 1 and autotune.batch_bucket(5) == 16
+    assert autotune.key(nat.C64, 256, 512, 3) == autotune.key('c64', 256, 512, 4) == '1:256x512:4'
+    assert autotune.lookup('c128', 64, 64, 1) in (None, tuple(autotune.lookup('c128', 64, 64, 1) or ()))
+    autotune.store('c128', 64, 64, 1, (4, 4, 8, 8), {'row': {'4x4': 1.0}, 'col': {}})
+    assert autotune.lookup(nat.C128, 64, 64, 1) == (4, 4, 8, 8)
+    # every listed shape divides its length into power-of-two thread counts
+    for pass_ in (0, 1):
+        for prec in ('c128', 'c64'):
+            for n in (64, 128, 256, 512, 1024, 2048, 4096):
+                shp = autotune.shapes(pass_, prec, n)
+                assert shp, (pass_, prec, n)
+                for e, p in shp:
+                    threads = p * n // e
+                    assert n % e == 0 and threads % 64 == 0 and threads <= 1024
