@@ -140,7 +140,8 @@ def main():
     ap.add_argument('--profile-steps', type=int, default=200)
     ap.add_argument('--cpu-budget', type=float, default=15.0, help='seconds of CPU baseline sampling')
     ap.add_argument('--no-cpu-baseline', action='store_true')
-    ap.add_argument('--no-tune', action='store_true', help='use cached/default FFT shapes')
+    ap.add_argument('--tune', action='store_true',
+                    help='re-measure the FFT shapes of the workload instead of using the shipped table')
     ap.add_argument('--precisions', nargs='+', default=['c128', 'c64'], choices=['c128', 'c64'])
     ap.add_argument('--calibrate-copy', type=int, default=0, metavar='MIB',
                     help='before timing, copy a MIB-sized device buffer once (a known-size '
@@ -154,7 +155,8 @@ def main():
         print("bench.py: no HIP device available", file=sys.stderr)
         sys.exit(1)
     os.environ.setdefault('SPINOR_GPE_DATA', tempfile.mkdtemp(prefix='sgpe_bench_'))
-    os.environ.setdefault('SGPE_TUNE_FILE', os.path.join(tempfile.mkdtemp(prefix='sgpe_tune_'), 'tune.json'))
+    if args.tune:
+        os.environ['SGPE_TUNE_FILE'] = os.path.join(tempfile.mkdtemp(prefix='sgpe_tune_'), 'tune.json')
     device = 'cuda:0'
     if args.calibrate_copy:
         src = torch.ones(args.calibrate_copy * 2 ** 20 // 4, dtype=torch.float32, device=device)
@@ -170,7 +172,7 @@ def main():
     for prec in args.precisions:
         log(f"timing {prec}: warmup {args.warmup}, steps {args.steps}")
         results[prec] = time_engine(ps, device, prec, args.steps, args.warmup, args.block,
-                                    args.profile_steps, not args.no_tune)
+                                    args.profile_steps, args.tune)
     r = results['c128']
     sps = args.steps / r['elapsed']
     bytes_c = {'c128': 16, 'c64': 8}

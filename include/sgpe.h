@@ -176,9 +176,10 @@ int sgpe_full_steps(sgpe_plan* plan, int n_steps);
 int sgpe_graph_steps(sgpe_plan* plan, int steps_per_graph, int n_graphs);
 
 /* Instrumented form of sgpe_full_steps for measurement: the same launches,
- * issued eagerly with a hipEvent recorded after each kernel on the context
- * stream, then synchronized. out4 (host) = {column-pass ms, row-pass ms,
- * column launches, row launches}. It serves the per-step timing methodology
+ * issued eagerly on the context stream, each carrying a start/stop hipEvent
+ * pair (hipExtLaunchKernelGGL) that timestamps the kernel's execution; then
+ * synchronized. out4 (host) = {column-pass ms, row-pass ms, column launches,
+ * row launches}. It serves the per-step timing methodology
  * of the reference's benchmark (spinor_gpe/benchmarks/benchmark_prop.py:
  * 69-79), which times prop.full_step() calls. */
 int sgpe_profile_full_steps(sgpe_plan* plan, int n_steps, double* out4);

@@ -72,7 +72,7 @@ static const KEntry* find_entry(int kind, int prec, int n, int e, int p) {
   return nullptr;
 }
 
-static const size_t kMaxLds = 160 * 1024;
+static const size_t kMaxLds = kLdsBytes;
 
 static size_t csize(int prec) { return prec == SGPE_C128 ? 16 : 8; }
 
@@ -114,14 +114,17 @@ struct sgpe_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipStream_t cap_stream = nullptr;  // private stream used only for graph capture
-  std::map<int, void*> tw;           // key prec*65536 + n
+  std::map<int, void*> tw;           // stage-major twiddles, key (prec*8192 + n)*32 + e
   double* red = nullptr;             // reduction scratch for primitives
   size_t red_count = 0;
   double* red_host = nullptr;        // pinned
 };
 
-static int ctx_twiddles(sgpe_ctx* ctx, int prec, int n, const void** out) {
-  const int key = prec * 65536 + n;
+// Stage-major twiddle table of an n-point transform done with e elements per
+// thread (radices e, e, ..., remainder): the stage of span L and radix R keeps
+// w_{L R}^{r k} (k < L, 1 <= r < R) at (L - 1) + k (R - 1) + (r - 1).
+static int ctx_twiddles(sgpe_ctx* ctx, int prec, int n, int e, const void** out) {
+  const int key = (prec * 8192 + n) * 32 + e;
   auto it = ctx->tw.find(key);
   if (it != ctx->tw.end()) {
     *out = it->second;
@@ -129,18 +132,29 @@ static int ctx_twiddles(sgpe_ctx* ctx, int prec, int n, const void** out) {
   }
   std::vector<double> re, im;
   twiddles_f64(n, re, im);
+  std::vector<double> sre(n, 0.0), sim(n, 0.0);
+  for (int L = 1; L < n;) {
+    const int R = (n / L >= e) ? e : n / L;
+    for (int k = 0; k < L; ++k)
+      for (int r = 1; r < R; ++r) {
+        const int m = r * k * (n / (L * R));
+        sre[(L - 1) + k * (R - 1) + (r - 1)] = re[m];
+        sim[(L - 1) + k * (R - 1) + (r - 1)] = im[m];
+      }
+    L *= R;
+  }
   std::vector<char> host(n * csize(prec));
   if (prec == SGPE_C128) {
     double* h = (double*)host.data();
     for (int i = 0; i < n; ++i) {
-      h[2 * i] = re[i];
-      h[2 * i + 1] = im[i];
+      h[2 * i] = sre[i];
+      h[2 * i + 1] = sim[i];
     }
   } else {
     float* h = (float*)host.data();
     for (int i = 0; i < n; ++i) {
-      h[2 * i] = (float)re[i];
-      h[2 * i + 1] = (float)im[i];
+      h[2 * i] = (float)sre[i];
+      h[2 * i + 1] = (float)sim[i];
     }
   }
   void* d = nullptr;
@@ -150,6 +164,12 @@ static int ctx_twiddles(sgpe_ctx* ctx, int prec, int n, const void** out) {
   ctx->tw[key] = d;
   *out = d;
   return SGPE_OK;
+}
+
+// Dynamic LDS of a kernel with data_bytes of exchange buffers for n-point
+// transforms: the twiddle table joins it when it fits (tw_in_lds).
+static size_t lds_bytes(size_t data_bytes, int n, size_t cs) {
+  return std::max<size_t>(data_bytes + (tw_in_lds(data_bytes, n, cs) ? (size_t)n * cs : 0), 256);
 }
 
 static int ctx_reduction(sgpe_ctx* ctx, size_t count) {
@@ -188,6 +208,8 @@ struct sgpe_plan {
   const KEntry* ecol = nullptr;
   const KEntry* erow = nullptr;
   int e_col_e = 8, e_col_w = 4, e_row_e = 8, e_row_rows = 1;
+  const void* etw_x = nullptr;  // twiddles of the energy kernels' shapes
+  const void* etw_y = nullptr;
 };
 
 static void drop_graphs(sgpe_plan* p) {
@@ -202,13 +224,19 @@ static int plan_select(sgpe_plan* p, int row_e, int row_rows, int col_e, int col
   const KEntry* c = find_entry(K_COL, p->prec, p->ny, col_e, col_w);
   if (!r || !c) return fail(SGPE_E_UNSUPPORTED, "FFT shape not compiled for this grid/precision");
   if (p->ny % row_rows || p->nx % col_w) return fail(SGPE_E_UNSUPPORTED, "tile does not divide the grid");
-  const size_t row_lds = (size_t)row_rows * 2 * p->nx * csize(p->prec);
-  const size_t col_lds = (size_t)col_w * p->ny * csize(p->prec);
-  if (row_lds > kMaxLds || col_lds > kMaxLds) return fail(SGPE_E_UNSUPPORTED, "FFT shape exceeds LDS");
-  int rc = set_lds_attr(r->kernel, row_lds);
+  const size_t cs = csize(p->prec);
+  const size_t row_data = (size_t)row_rows * 2 * p->nx * cs;
+  const size_t col_data = (size_t)col_w * p->ny * cs;
+  if (row_data > kMaxLds || col_data > kMaxLds) return fail(SGPE_E_UNSUPPORTED, "FFT shape exceeds LDS");
+  int rc = set_lds_attr(r->kernel, lds_bytes(row_data, p->nx, cs));
   if (rc) return rc;
-  rc = set_lds_attr(c->kernel, col_lds);
+  rc = set_lds_attr(c->kernel, lds_bytes(col_data, p->ny, cs));
   if (rc) return rc;
+  const void *twx = nullptr, *twy = nullptr;
+  if ((rc = ctx_twiddles(p->ctx, p->prec, p->nx, row_e, &twx))) return rc;
+  if ((rc = ctx_twiddles(p->ctx, p->prec, p->ny, col_e, &twy))) return rc;
+  p->tw_x = twx;
+  p->tw_y = twy;
   p->row = r;
   p->col = c;
   p->row_e = row_e;
@@ -219,16 +247,18 @@ static int plan_select(sgpe_plan* p, int row_e, int row_rows, int col_e, int col
   return SGPE_OK;
 }
 
-static void launch_col(sgpe_plan* p, hipStream_t s, int mode, int dprev, int dcur) {
-  const size_t lds = std::max<size_t>((size_t)p->col_w * p->ny * csize(p->prec), 256);
+static void launch_col(sgpe_plan* p, hipStream_t s, int mode, int dprev, int dcur, hipEvent_t e0 = nullptr,
+                       hipEvent_t e1 = nullptr) {
+  const size_t lds = lds_bytes((size_t)p->col_w * p->ny * csize(p->prec), p->ny, csize(p->prec));
   dim3 grid(p->nx / p->col_w, 2, p->nslots), block(p->col_w * (p->ny / p->col_e));
-  ((ColFn)p->col->launch)(grid, block, lds, s, p->d_inst, p->d_run, p->tw_y, mode, dprev, dcur);
+  ((ColFn)p->col->launch)(grid, block, lds, s, e0, e1, p->d_inst, p->d_run, p->tw_y, mode, dprev, dcur);
 }
 
-static void launch_row(sgpe_plan* p, hipStream_t s, int dur, int first) {
-  const size_t lds = std::max<size_t>((size_t)p->row_rows * 2 * p->nx * csize(p->prec), 256);
+static void launch_row(sgpe_plan* p, hipStream_t s, int dur, int first, hipEvent_t e0 = nullptr,
+                       hipEvent_t e1 = nullptr) {
+  const size_t lds = lds_bytes((size_t)p->row_rows * 2 * p->nx * csize(p->prec), p->nx, csize(p->prec));
   dim3 grid(p->ny / p->row_rows, 1, p->nslots), block(p->row_rows * (p->nx / p->row_e));
-  ((RowFn)p->row->launch)(grid, block, lds, s, p->d_inst, p->d_run, p->tw_x, dur, first, p->col_w);
+  ((RowFn)p->row->launch)(grid, block, lds, s, e0, e1, p->d_inst, p->d_run, p->tw_x, dur, first, p->col_w);
 }
 
 static void issue_sub_step(sgpe_plan* p, hipStream_t s, int which, int first) {
@@ -280,8 +310,10 @@ static int plan_energy_prepare(sgpe_plan* p) {
   p->ecol = find_entry(K_ECOL, p->prec, p->ny, p->e_col_e, p->e_col_w);
   p->erow = find_entry(K_EROW, p->prec, p->nx, p->e_row_e, p->e_row_rows);
   if (!p->ecol || !p->erow) return fail(SGPE_E_UNSUPPORTED, "energy kernels not compiled for this grid");
-  if ((rc = set_lds_attr(p->ecol->kernel, (size_t)p->e_col_w * p->ny * cs))) return rc;
-  if ((rc = set_lds_attr(p->erow->kernel, (size_t)p->e_row_rows * 2 * p->nx * cs))) return rc;
+  if ((rc = set_lds_attr(p->ecol->kernel, lds_bytes((size_t)p->e_col_w * p->ny * cs, p->ny, cs)))) return rc;
+  if ((rc = set_lds_attr(p->erow->kernel, lds_bytes((size_t)p->e_row_rows * 2 * p->nx * cs, p->nx, cs)))) return rc;
+  if ((rc = ctx_twiddles(p->ctx, p->prec, p->ny, p->e_col_e, &p->etw_y))) return rc;
+  if ((rc = ctx_twiddles(p->ctx, p->prec, p->nx, p->e_row_e, &p->etw_x))) return rc;
   if (hipMalloc(&p->scratch, per * p->nslots) != hipSuccess) {
     p->scratch = nullptr;
     return fail(SGPE_E_NOMEM, "energy work buffer allocation failed");
@@ -297,12 +329,12 @@ static int plan_energy_prepare(sgpe_plan* p) {
 static void issue_energy(sgpe_plan* p, hipStream_t s, double* out) {
   issue_sync(p, s);
   const size_t cs = csize(p->prec);
-  const size_t cl = std::max<size_t>((size_t)p->e_col_w * p->ny * cs, 256);
-  const size_t rl = std::max<size_t>((size_t)p->e_row_rows * 2 * p->nx * cs, 256);
+  const size_t cl = lds_bytes((size_t)p->e_col_w * p->ny * cs, p->ny, cs);
+  const size_t rl = lds_bytes((size_t)p->e_row_rows * 2 * p->nx * cs, p->nx, cs);
   dim3 cg(p->nx / p->e_col_w, 2, p->nslots), cb(p->e_col_w * (p->ny / p->e_col_e));
   dim3 rg(p->ny / p->e_row_rows, 1, p->nslots), rb(p->e_row_rows * (p->nx / p->e_row_e));
-  ((EFn)p->ecol->launch)(cg, cb, cl, s, p->d_inst, p->d_run, p->tw_y);
-  ((EFn)p->erow->launch)(rg, rb, rl, s, p->d_inst, p->d_run, p->tw_x);
+  ((EFn)p->ecol->launch)(cg, cb, cl, s, p->d_inst, p->d_run, p->etw_y);
+  ((EFn)p->erow->launch)(rg, rb, rl, s, p->d_inst, p->d_run, p->etw_x);
   hipLaunchKernelGGL(energy_final_kernel, dim3(p->nslots), dim3(256), 0, s, p->d_inst, p->d_run,
                      p->nx / p->e_col_w, p->ny / p->e_row_rows, out);
 }
@@ -587,14 +619,14 @@ int sgpe_fft2(sgpe_ctx* ctx, int precision, int ny, int nx, int n_arrays, void* 
   const KEntry* fr = find_entry(K_FROW, precision, nx, re, rr);
   const KEntry* fc = find_entry(K_FCOL, precision, ny, ce, cw);
   if (!fr || !fc) return fail(SGPE_E_UNSUPPORTED, "FFT shape not compiled");
-  const size_t rl = std::max<size_t>((size_t)rr * nx * csize(precision), 256);
-  const size_t cl = std::max<size_t>((size_t)cw * ny * csize(precision), 256);
+  const size_t rl = lds_bytes((size_t)rr * nx * csize(precision), nx, csize(precision));
+  const size_t cl = lds_bytes((size_t)cw * ny * csize(precision), ny, csize(precision));
   int rc = set_lds_attr(fr->kernel, rl);
   if (rc) return rc;
   if ((rc = set_lds_attr(fc->kernel, cl))) return rc;
   const void *twx, *twy;
-  if ((rc = ctx_twiddles(ctx, precision, nx, &twx))) return rc;
-  if ((rc = ctx_twiddles(ctx, precision, ny, &twy))) return rc;
+  if ((rc = ctx_twiddles(ctx, precision, nx, re, &twx))) return rc;
+  if ((rc = ctx_twiddles(ctx, precision, ny, ce, &twy))) return rc;
   const double c = dx * dy / (2.0 * M_PI);
   dim3 rg(ny / rr), rb(rr * (nx / re)), cg(nx / cw), cb(cw * (ny / ce));
   for (int i = 0; i < n_arrays; ++i) {
@@ -691,8 +723,6 @@ int sgpe_plan_create(sgpe_ctx* ctx, int precision, int ny, int nx, int n_slots, 
   default_row(precision, nx, &re, &rr);
   default_col(precision, ny, &ce, &cw);
   int rc = plan_select(p, re, rr, ce, cw);
-  if (!rc) rc = ctx_twiddles(ctx, precision, nx, &p->tw_x);
-  if (!rc) rc = ctx_twiddles(ctx, precision, ny, &p->tw_y);
   if (!rc && hipMalloc((void**)&p->d_inst, n_slots * sizeof(InstDev)) != hipSuccess)
     rc = fail(SGPE_E_NOMEM, "slot table allocation failed");
   if (!rc && hipMalloc((void**)&p->d_run, n_slots * sizeof(RunDev)) != hipSuccess)
@@ -870,24 +900,23 @@ int sgpe_profile_full_steps(sgpe_plan* p, int n_steps, double* out4) {
   HIP_TRY(hipSetDevice(p->ctx->device));
   hipStream_t s = p->ctx->stream;
   const int n_launch = 6 * n_steps;
-  std::vector<hipEvent_t> ev(n_launch + 1, nullptr);
+  std::vector<hipEvent_t> ev(2 * n_launch, nullptr);
   for (auto& e : ev)
     if (hipEventCreate(&e) != hipSuccess) {
       for (auto& f : ev)
         if (f) (void)hipEventDestroy(f);
       return fail(SGPE_E_HIP, "event creation failed");
     }
-  // Same launches as sgpe_full_steps, with an event between consecutive
-  // kernels: launch 2i is a column pass, 2i+1 a row pass.
-  (void)hipEventRecord(ev[0], s);
+  // The launches of sgpe_full_steps, each carrying a start/stop event pair
+  // that timestamps the kernel itself (no extra packets between kernels).
   int k = 0;
   for (int i = 0; i < n_steps; ++i) {
     const int durs[3] = {DUR_OUT, DUR_IN, DUR_OUT};
     for (int j = 0; j < 3; ++j) {
-      launch_col(p, s, COL_MID, p->last_dur, durs[j]);
-      (void)hipEventRecord(ev[++k], s);
-      launch_row(p, s, durs[j], j == 0);
-      (void)hipEventRecord(ev[++k], s);
+      launch_col(p, s, COL_MID, p->last_dur, durs[j], ev[2 * k], ev[2 * k + 1]);
+      ++k;
+      launch_row(p, s, durs[j], j == 0, ev[2 * k], ev[2 * k + 1]);
+      ++k;
       p->last_dur = durs[j];
     }
   }
@@ -896,7 +925,7 @@ int sgpe_profile_full_steps(sgpe_plan* p, int n_steps, double* out4) {
   double col = 0.0, row = 0.0;
   for (int i = 0; i < n_launch && le == hipSuccess && se == hipSuccess; ++i) {
     float ms = 0.f;
-    if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) != hipSuccess) break;
+    if (hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]) != hipSuccess) break;
     if (i % 2 == 0)
       col += ms;
     else

@@ -10,13 +10,25 @@
 // loads/stores are unit-stride across threads (coalesced).
 //
 // Unnormalized: forward uses exp(-2 pi i/N), inverse exp(+2 pi i/N), no 1/N.
-// Twiddles come from a host-built table tw[m] = exp(-2 pi i m/N) (rounded
-// from float64 values computed with exact octant reduction).
+// Twiddles come from a host-built stage-major table (rounded from float64
+// values computed with exact octant reduction): the stage of span L and
+// radix R keeps w_{L R}^{r k} for k < L, 1 <= r < R at (L - 1) + k (R - 1) +
+// (r - 1), so a butterfly reads R - 1 consecutive entries and the table has
+// N - 1 entries in all. Kernels copy it to LDS when it fits next to their
+// data (tw_in_lds) and read it from global memory otherwise.
 #pragma once
 
 #include "sgpe_common.h"
 
 namespace sgpe {
+
+constexpr size_t kLdsBytes = 160 * 1024;
+
+// Whether a kernel with data_bytes of LDS exchange buffers also holds the
+// n-entry twiddle table in LDS (same predicate on host and device).
+constexpr bool tw_in_lds(size_t data_bytes, int n, size_t elem_bytes) {
+  return data_bytes + (size_t)n * elem_bytes <= kLdsBytes;
+}
 
 // Multiply by -i (forward) or +i (inverse).
 template <bool INV, typename C> __device__ __forceinline__ C rot90(C a) {
@@ -119,7 +131,7 @@ template <int STRIDE, int PADB, int PADW> struct LdsView {
 // One Stockham stage. Non-final stages write their outputs to LDS; the final
 // stage leaves output Y[t + j*T] in a[j].
 template <int N, int E, int R, int L, bool INV, bool LAST, class V, typename C>
-__device__ __forceinline__ void fft_stage(C (&a)[E], int t, const C* __restrict__ tw, C* lds) {
+__device__ __forceinline__ void fft_stage(C (&a)[E], int t, const C* tw, C* lds) {
   constexpr int T = N / E;
   constexpr int U = E / R;
   C out[LAST ? E : 1];
@@ -131,10 +143,10 @@ __device__ __forceinline__ void fft_stage(C (&a)[E], int t, const C* __restrict_
 #pragma unroll
     for (int r = 0; r < R; ++r) b[r] = a[u + r * U];
     if constexpr (L > 1) {
-      constexpr int step = N / (L * R);
+      const C* wk = tw + (L - 1) + k * (R - 1) - 1;
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        C w = tw[r * k * step];
+        C w = wk[r];
         if (INV) w.y = -w.y;
         b[r] = cmul(b[r], w);
       }
@@ -158,7 +170,7 @@ __device__ __forceinline__ void fft_stage(C (&a)[E], int t, const C* __restrict_
 // Full transform: stages of radix E while N/L >= E, then the remainder radix.
 // Every thread of the block must call this (it contains barriers).
 template <int N, int E, bool INV, class V, int L = 1, typename C>
-__device__ __forceinline__ void fft_run(C (&a)[E], int t, const C* __restrict__ tw, C* lds) {
+__device__ __forceinline__ void fft_run(C (&a)[E], int t, const C* tw, C* lds) {
   constexpr int R = (N / L >= E) ? E : N / L;
   constexpr bool LAST = (L * R == N);
   constexpr int T = N / E;
@@ -175,8 +187,7 @@ __device__ __forceinline__ void fft_run(C (&a)[E], int t, const C* __restrict__ 
 // Two independent transforms (e.g. both spinor components of one row) that
 // share barriers: lds0/lds1 are disjoint views.
 template <int N, int E, bool INV, class V, int L = 1, typename C>
-__device__ __forceinline__ void fft_run2(C (&a)[E], C (&b)[E], int t, const C* __restrict__ tw, C* lds0,
-                                         C* lds1) {
+__device__ __forceinline__ void fft_run2(C (&a)[E], C (&b)[E], int t, const C* tw, C* lds0, C* lds1) {
   constexpr int R = (N / L >= E) ? E : N / L;
   constexpr bool LAST = (L * R == N);
   constexpr int T = N / E;
@@ -191,6 +202,21 @@ __device__ __forceinline__ void fft_run2(C (&a)[E], C (&b)[E], int t, const C* _
     }
     __syncthreads();
     fft_run2<N, E, INV, V, L * R>(a, b, t, tw, lds0, lds1);
+  }
+}
+
+// Twiddle table for the FFT stages: copies the n - 1 stage-major entries to
+// LDS at `dst` when IN_LDS (the first exchange barrier of fft_run orders the
+// copy before any read) and returns the table to read from.
+template <bool IN_LDS, typename C>
+__device__ __forceinline__ const C* stage_twiddles(const C* __restrict__ tw, int n, C* dst) {
+  if constexpr (IN_LDS) {
+    for (int i = threadIdx.x; i < n - 1; i += blockDim.x) dst[i] = tw[i];
+    return dst;
+  } else {
+    (void)n;
+    (void)dst;
+    return tw;
   }
 }
 

@@ -121,7 +121,7 @@ template <typename R> __device__ __forceinline__ double norm2d(typename Cx<R>::T
 template <typename R, int N, int E, int W>
 __global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __restrict__ insts,
                                                              RunDev* __restrict__ runs,
-                                                             const typename Cx<R>::T* __restrict__ tw,
+                                                             const typename Cx<R>::T* __restrict__ tws,
                                                              int mode, int d_prev, int d_cur) {
   typedef typename Cx<R>::T C;
   typedef LdsView<W, 1 << 30, 0> V;
@@ -139,6 +139,8 @@ __global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __re
   const int jx = tile * W + c;
   C* lds = reinterpret_cast<C*>(smem) + c;
   C* col = reinterpret_cast<C*>(P.psi) + (size_t)comp * N * nx + jx;
+  constexpr size_t kData = (size_t)W * N * sizeof(C);
+  const C* tw = stage_twiddles<tw_in_lds(kData, N, sizeof(C))>(tws, N, reinterpret_cast<C*>(smem) + W * N);
 
   C a[E];
 #pragma unroll
@@ -205,7 +207,7 @@ __device__ __forceinline__ void part_sums(const double* part, int nt2, int slot,
 template <typename R, int N, int E, int ROWS>
 __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* __restrict__ insts,
                                                                 RunDev* __restrict__ runs,
-                                                                const typename Cx<R>::T* __restrict__ tw,
+                                                                const typename Cx<R>::T* __restrict__ tws,
                                                                 int dur, int first_sub, int col_w) {
   typedef typename Cx<R>::T C;
   typedef LdsView<1, 1 << 30, 0> V;
@@ -217,6 +219,8 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
   const InstDev& P = insts[inst];
   const int ny = P.ny;
   const int nt2 = 2 * (N / col_w);
+  constexpr size_t kData = (size_t)ROWS * 2 * N * sizeof(C);
+  const C* tw = stage_twiddles<tw_in_lds(kData, N, sizeof(C))>(tws, N, reinterpret_cast<C*>(smem) + ROWS * 2 * N);
 
   // Real-space normalization (tensor_tools.py:301-305) from the column
   // pass's k-space sums by Parseval: sum_r |IDFT2 v|^2 = Nx Ny sum_k |v|^2.
@@ -417,7 +421,7 @@ static __global__ void config_run_kernel(RunDev* __restrict__ runs, int slot, do
 template <typename R, int N, int E, int W>
 __global__ __launch_bounds__(W*(N / E)) void energy_col_kernel(const InstDev* __restrict__ insts,
                                                                const RunDev* __restrict__ runs,
-                                                               const typename Cx<R>::T* __restrict__ tw) {
+                                                               const typename Cx<R>::T* __restrict__ tws) {
   typedef typename Cx<R>::T C;
   typedef LdsView<W, 1 << 30, 0> V;
   constexpr int T = N / E;
@@ -434,6 +438,8 @@ __global__ __launch_bounds__(W*(N / E)) void energy_col_kernel(const InstDev* __
   C* lds = reinterpret_cast<C*>(smem) + c;
   const C* col = reinterpret_cast<const C*>(P.psi) + (size_t)comp * N * nx + jx;
   C* out = reinterpret_cast<C*>(P.scratch) + (size_t)comp * N * nx + jx;
+  constexpr size_t kData = (size_t)W * N * sizeof(C);
+  const C* tw = stage_twiddles<tw_in_lds(kData, N, sizeof(C))>(tws, N, reinterpret_cast<C*>(smem) + W * N);
   C a[E];
 #pragma unroll
   for (int j = 0; j < E; ++j) a[j] = col[(size_t)(t + j * T) * nx];
@@ -464,7 +470,7 @@ __global__ __launch_bounds__(W*(N / E)) void energy_col_kernel(const InstDev* __
 template <typename R, int N, int E, int ROWS>
 __global__ __launch_bounds__(ROWS*(N / E)) void energy_row_kernel(const InstDev* __restrict__ insts,
                                                                   const RunDev* __restrict__ runs,
-                                                                  const typename Cx<R>::T* __restrict__ tw) {
+                                                                  const typename Cx<R>::T* __restrict__ tws) {
   typedef typename Cx<R>::T C;
   typedef LdsView<1, 1 << 30, 0> V;
   constexpr int T = N / E;
@@ -480,6 +486,8 @@ __global__ __launch_bounds__(ROWS*(N / E)) void energy_row_kernel(const InstDev*
   const C* r1 = r0 + (size_t)ny * N;
   C* lds0 = reinterpret_cast<C*>(smem) + (size_t)row * 2 * N;
   C* lds1 = lds0 + N;
+  constexpr size_t kData = (size_t)ROWS * 2 * N * sizeof(C);
+  const C* tw = stage_twiddles<tw_in_lds(kData, N, sizeof(C))>(tws, N, reinterpret_cast<C*>(smem) + ROWS * 2 * N);
   C a0[E], a1[E];
 #pragma unroll
   for (int j = 0; j < E; ++j) {
@@ -587,7 +595,7 @@ template <typename R, int N, int E, int ROWS>
 __global__ __launch_bounds__(ROWS*(N / E)) void fft_rows_kernel(const typename Cx<R>::T* __restrict__ in,
                                                                 typename Cx<R>::T* __restrict__ out, int inv,
                                                                 int shift, double scale,
-                                                                const typename Cx<R>::T* __restrict__ tw) {
+                                                                const typename Cx<R>::T* __restrict__ tws) {
   typedef typename Cx<R>::T C;
   typedef LdsView<1, 1 << 30, 0> V;
   constexpr int T = N / E;
@@ -598,6 +606,8 @@ __global__ __launch_bounds__(ROWS*(N / E)) void fft_rows_kernel(const typename C
   const C* src = in + (size_t)y * N;
   C* dst = out + (size_t)y * N;
   C* lds = reinterpret_cast<C*>(smem) + (size_t)row * N;
+  constexpr size_t kData = (size_t)ROWS * N * sizeof(C);
+  const C* tw = stage_twiddles<tw_in_lds(kData, N, sizeof(C))>(tws, N, reinterpret_cast<C*>(smem) + ROWS * N);
   C a[E];
 #pragma unroll
   for (int j = 0; j < E; ++j) a[j] = src[t + j * T];
@@ -619,7 +629,7 @@ __global__ __launch_bounds__(ROWS*(N / E)) void fft_rows_kernel(const typename C
 template <typename R, int N, int E, int W>
 __global__ __launch_bounds__(W*(N / E)) void fft_cols_kernel(const typename Cx<R>::T* __restrict__ in,
                                                              typename Cx<R>::T* __restrict__ out, int nx,
-                                                             int inv, const typename Cx<R>::T* __restrict__ tw) {
+                                                             int inv, const typename Cx<R>::T* __restrict__ tws) {
   typedef typename Cx<R>::T C;
   typedef LdsView<W, 1 << 30, 0> V;
   constexpr int T = N / E;
@@ -628,6 +638,8 @@ __global__ __launch_bounds__(W*(N / E)) void fft_cols_kernel(const typename Cx<R
   const int t = threadIdx.x / W;
   const int jx = blockIdx.x * W + c;
   C* lds = reinterpret_cast<C*>(smem) + c;
+  constexpr size_t kData = (size_t)W * N * sizeof(C);
+  const C* tw = stage_twiddles<tw_in_lds(kData, N, sizeof(C))>(tws, N, reinterpret_cast<C*>(smem) + W * N);
   C a[E];
 #pragma unroll
   for (int j = 0; j < E; ++j) a[j] = in[(size_t)(t + j * T) * nx + jx];

@@ -3,12 +3,18 @@
 // they compile in parallel; sgpe_api.hip looks shapes up here.
 #pragma once
 
+#include <hip/hip_ext.h>
+
 #include "sgpe_kernels.h"
 
 namespace sgpe {
 
-typedef void (*ColFn)(dim3, dim3, size_t, hipStream_t, const InstDev*, RunDev*, const void*, int, int, int);
-typedef void (*RowFn)(dim3, dim3, size_t, hipStream_t, const InstDev*, RunDev*, const void*, int, int, int);
+// Split-step pass launchers; with non-null events the launch carries them
+// (hipExtLaunchKernelGGL), which timestamps the kernel's own execution.
+typedef void (*ColFn)(dim3, dim3, size_t, hipStream_t, hipEvent_t, hipEvent_t, const InstDev*, RunDev*, const void*,
+                      int, int, int);
+typedef void (*RowFn)(dim3, dim3, size_t, hipStream_t, hipEvent_t, hipEvent_t, const InstDev*, RunDev*, const void*,
+                      int, int, int);
 typedef void (*FRowFn)(dim3, dim3, size_t, hipStream_t, const void*, void*, int, int, double, const void*);
 typedef void (*FColFn)(dim3, dim3, size_t, hipStream_t, const void*, void*, int, int, const void*);
 typedef void (*EFn)(dim3, dim3, size_t, hipStream_t, const InstDev*, const RunDev*, const void*);
@@ -30,15 +36,24 @@ KTable registry_f64(int kind);
 KTable registry_f32(int kind);
 
 template <typename R, int N, int E, int W>
-void col_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const InstDev* i, RunDev* r, const void* tw, int m, int dp,
-                int dc) {
-  hipLaunchKernelGGL((col_pass_kernel<R, N, E, W>), g, b, l, s, i, r, (const typename Cx<R>::T*)tw, m, dp, dc);
+void col_launch(dim3 g, dim3 b, size_t l, hipStream_t s, hipEvent_t e0, hipEvent_t e1, const InstDev* i, RunDev* r,
+                const void* tw, int m, int dp, int dc) {
+  typedef typename Cx<R>::T C;
+  if (e0)
+    hipExtLaunchKernelGGL((col_pass_kernel<R, N, E, W>), g, b, (uint32_t)l, s, e0, e1, 0, i, r, (const C*)tw, m, dp,
+                          dc);
+  else
+    hipLaunchKernelGGL((col_pass_kernel<R, N, E, W>), g, b, l, s, i, r, (const C*)tw, m, dp, dc);
 }
 template <typename R, int N, int E, int ROWS>
-void row_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const InstDev* i, RunDev* r, const void* tw, int dur,
-                int first, int colw) {
-  hipLaunchKernelGGL((row_pass_kernel<R, N, E, ROWS>), g, b, l, s, i, r, (const typename Cx<R>::T*)tw, dur, first,
-                     colw);
+void row_launch(dim3 g, dim3 b, size_t l, hipStream_t s, hipEvent_t e0, hipEvent_t e1, const InstDev* i, RunDev* r,
+                const void* tw, int dur, int first, int colw) {
+  typedef typename Cx<R>::T C;
+  if (e0)
+    hipExtLaunchKernelGGL((row_pass_kernel<R, N, E, ROWS>), g, b, (uint32_t)l, s, e0, e1, 0, i, r, (const C*)tw, dur,
+                          first, colw);
+  else
+    hipLaunchKernelGGL((row_pass_kernel<R, N, E, ROWS>), g, b, l, s, i, r, (const C*)tw, dur, first, colw);
 }
 template <typename R, int N, int E, int ROWS>
 void frow_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const void* in, void* out, int inv, int shift, double scale,

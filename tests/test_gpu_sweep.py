@@ -26,8 +26,14 @@ def instances(n, seed=11):
     return out
 
 
+@pytest.mark.parametrize('tuned', [False, True])
 @pytest.mark.parametrize('precision', ['c128', 'c64'])
-def test_sweep_alone_vs_batch(gpu, precision):
+def test_sweep_alone_vs_batch(gpu, precision, tuned, monkeypatch):
+    """Same FFT shapes (tuning cache off): bit-identical alone and in the mix.
+    With the tuner's per-batch-size shapes the rounding may differ: then the
+    states agree to 1e-12 (c128) / 2e-4 (c64) relative and steps match."""
+    if not tuned:
+        monkeypatch.setenv('SGPE_NO_TUNE_CACHE', '1')
     spins = instances(6)
     batch = Sweep(gpu, precision, conv_tol=1e-7, check_every=25, max_steps=1500)
     for ps in spins:
@@ -41,9 +47,15 @@ def test_sweep_alone_vs_batch(gpu, precision):
         alone.add_spin(ps, 0.02)
         (one,), _ = alone.run()
         assert one.steps == res[i].steps and one.converged == res[i].converged
-        # identical kernels and reduction order: the batch changes nothing
-        np.testing.assert_array_equal(one.psik, res[i].psik)
-        assert one.energy == res[i].energy
+        if not tuned:
+            # identical kernels and reduction order: the batch changes nothing
+            np.testing.assert_array_equal(one.psik, res[i].psik)
+            assert one.energy == res[i].energy
+        else:
+            tol = 1e-12 if precision == 'c128' else 2e-4
+            d = np.linalg.norm(one.psik - res[i].psik) / np.linalg.norm(one.psik)
+            assert d < tol, d
+            assert one.energy['E_tot'] == pytest.approx(res[i].energy['E_tot'], rel=tol)
         assert res[i].energy['N'] == pytest.approx(ps.atom_num, rel=1e-6)
 
 
