@@ -167,10 +167,18 @@ static int ctx_twiddles(sgpe_ctx* ctx, int prec, int n, int e, const void** out)
 }
 
 // Dynamic LDS of a kernel with data_bytes of exchange buffers for n-point
-// transforms: the twiddle table joins it when it fits (tw_in_lds).
-static size_t lds_bytes(size_t data_bytes, int n, size_t cs) {
-  return std::max<size_t>(data_bytes + (tw_in_lds(data_bytes, n, cs) ? (size_t)n * cs : 0), 256);
+// transforms: the twiddle table joins it when it fits (tw_in_lds), and the
+// column pass adds its `tables` (the 2 kinetic y-tables) when those fit
+// too; the same predicates as in the kernels.
+static size_t lds_bytes(size_t data_bytes, int n, size_t cs, int tables = 0) {
+  size_t total = data_bytes;
+  if (tw_in_lds(data_bytes, n, cs)) {
+    total += (size_t)n * cs;
+    if (tables && data_bytes + (size_t)(1 + tables) * n * cs <= kLdsBytes) total += (size_t)tables * n * cs;
+  }
+  return std::max<size_t>(total, 256);
 }
+static const int kColTables = 2;
 
 static int ctx_reduction(sgpe_ctx* ctx, size_t count) {
   if (ctx->red_count >= count) return SGPE_OK;
@@ -230,7 +238,7 @@ static int plan_select(sgpe_plan* p, int row_e, int row_rows, int col_e, int col
   if (row_data > kMaxLds || col_data > kMaxLds) return fail(SGPE_E_UNSUPPORTED, "FFT shape exceeds LDS");
   int rc = set_lds_attr(r->kernel, lds_bytes(row_data, p->nx, cs));
   if (rc) return rc;
-  rc = set_lds_attr(c->kernel, lds_bytes(col_data, p->ny, cs));
+  rc = set_lds_attr(c->kernel, lds_bytes(col_data, p->ny, cs, kColTables));
   if (rc) return rc;
   const void *twx = nullptr, *twy = nullptr;
   if ((rc = ctx_twiddles(p->ctx, p->prec, p->nx, row_e, &twx))) return rc;
@@ -249,7 +257,7 @@ static int plan_select(sgpe_plan* p, int row_e, int row_rows, int col_e, int col
 
 static void launch_col(sgpe_plan* p, hipStream_t s, int mode, int dprev, int dcur, hipEvent_t e0 = nullptr,
                        hipEvent_t e1 = nullptr) {
-  const size_t lds = lds_bytes((size_t)p->col_w * p->ny * csize(p->prec), p->ny, csize(p->prec));
+  const size_t lds = lds_bytes((size_t)p->col_w * p->ny * csize(p->prec), p->ny, csize(p->prec), kColTables);
   dim3 grid(p->nx / p->col_w, 2, p->nslots), block(p->col_w * (p->ny / p->col_e));
   ((ColFn)p->col->launch)(grid, block, lds, s, e0, e1, p->d_inst, p->d_run, p->tw_y, mode, dprev, dcur);
 }

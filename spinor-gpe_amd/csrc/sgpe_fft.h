@@ -220,6 +220,41 @@ __device__ __forceinline__ const C* stage_twiddles(const C* __restrict__ tw, int
   }
 }
 
+// Two-phase copy of an N-entry table into LDS by a block of BLOCK threads:
+// load() issues the global loads into registers, store() writes them to LDS
+// later, so that the loads of a kernel's prologue are all in flight together.
+// With IN_LDS false it only forwards the global pointer. The caller makes the
+// stores visible with a barrier before the first read.
+template <bool IN_LDS, int N, int BLOCK, typename C> struct TableStage {
+  static constexpr int kPer = IN_LDS ? (N + BLOCK - 1) / BLOCK : 1;
+  C v[kPer];
+  const C* src = nullptr;
+  __device__ __forceinline__ void load(const C* __restrict__ s, int count = N) {
+    src = s;
+    if constexpr (IN_LDS) {
+#pragma unroll
+      for (int q = 0; q < kPer; ++q) {
+        const int i = threadIdx.x + q * BLOCK;
+        if (i < count) v[q] = s[i];
+      }
+    }
+  }
+  __device__ __forceinline__ const C* store(C* dst, int count = N) {
+    if constexpr (IN_LDS) {
+#pragma unroll
+      for (int q = 0; q < kPer; ++q) {
+        const int i = threadIdx.x + q * BLOCK;
+        if (i < count) dst[i] = v[q];
+      }
+      return dst;
+    } else {
+      (void)dst;
+      (void)count;
+      return src;
+    }
+  }
+};
+
 // Deterministic block-wide sum of NV doubles per thread (fixed tree order).
 // `scratch` needs NV * (blockDim.x / 64) doubles. All threads get the result.
 template <int NV>

@@ -126,6 +126,10 @@ __global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __re
   typedef typename Cx<R>::T C;
   typedef LdsView<W, 1 << 30, 0> V;
   constexpr int T = N / E;
+  // LDS: [W*N exchange][N twiddles][N ky_prev][N ky_cur] (tables when they fit)
+  constexpr size_t kData = (size_t)W * N * sizeof(C);
+  constexpr bool kTw = tw_in_lds(kData, N, sizeof(C));
+  constexpr bool kKy = kTw && kData + 3 * (size_t)N * sizeof(C) <= kLdsBytes;
   extern __shared__ double smem[];
   const int inst = blockIdx.z;
   if (!runs[inst].active) return;
@@ -137,29 +141,38 @@ __global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __re
   const int c = threadIdx.x % W;
   const int t = threadIdx.x / W;
   const int jx = tile * W + c;
-  C* lds = reinterpret_cast<C*>(smem) + c;
+  C* const sm = reinterpret_cast<C*>(smem);
+  C* lds = sm + c;
   C* col = reinterpret_cast<C*>(P.psi) + (size_t)comp * N * nx + jx;
-  constexpr size_t kData = (size_t)W * N * sizeof(C);
-  const C* tw = stage_twiddles<tw_in_lds(kData, N, sizeof(C))>(tws, N, reinterpret_cast<C*>(smem) + W * N);
 
+  // Every global load is issued before any computation: the state first,
+  // then the kinetic tables and twiddles (staged into LDS).
+  constexpr int kBlock = W * T;
   C a[E];
 #pragma unroll
   for (int j = 0; j < E; ++j) a[j] = col[(size_t)(t + j * T) * nx];
-
-  if (!kspace) fft_run<N, E, false, V>(a, t, tw, lds);
-
-  const bool imag = P.imag != 0;
   const double* td = P.t_dense[comp];
-  double s_acc = 0.0, n_acc = 0.0;
   C kxp = C{(R)1, (R)0}, kxc = C{(R)1, (R)0};
-  const C* kyp = nullptr;
-  const C* kyc = nullptr;
+  TableStage<kKy, N, kBlock, C> sp, sc_;
+  TableStage<kTw, N, kBlock, C> stw;
   if (!td) {
     kxp = reinterpret_cast<const C*>(P.kx[d_prev][comp])[jx];
-    kyp = reinterpret_cast<const C*>(P.ky[d_prev][comp]);
     kxc = reinterpret_cast<const C*>(P.kx[d_cur][comp])[jx];
-    kyc = reinterpret_cast<const C*>(P.ky[d_cur][comp]);
+    sp.load(reinterpret_cast<const C*>(P.ky[d_prev][comp]));
+    sc_.load(reinterpret_cast<const C*>(P.ky[d_cur][comp]));
   }
+  stw.load(tws, N - 1);
+  const C* kyp = td ? nullptr : sp.store(sm + W * N + N);
+  const C* kyc = td ? nullptr : sc_.store(sm + W * N + 2 * N);
+  const C* tw = stw.store(sm + W * N, N - 1);
+
+  if (!kspace)
+    fft_run<N, E, false, V>(a, t, tw, lds);
+  else
+    __syncthreads();  // publishes the staged tables
+
+  const bool imag = P.imag != 0;
+  double s_acc = 0.0, n_acc = 0.0;
   const R hp = (R)(0.5 * P.dt_sub[d_prev]);
   const R hc = (R)(0.5 * P.dt_sub[d_cur]);
 #pragma unroll
@@ -212,6 +225,10 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
   typedef typename Cx<R>::T C;
   typedef LdsView<1, 1 << 30, 0> V;
   constexpr int T = N / E;
+  // LDS: [ROWS*2*N exchange][N twiddles]. The x phase tables stay in
+  // global memory (cache resident): copies in LDS would cost occupancy.
+  constexpr size_t kData = (size_t)ROWS * 2 * N * sizeof(C);
+  constexpr bool kTw = tw_in_lds(kData, N, sizeof(C));
   extern __shared__ double smem[];
   const int inst = blockIdx.z;
   RunDev& run = runs[inst];
@@ -219,54 +236,25 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
   const InstDev& P = insts[inst];
   const int ny = P.ny;
   const int nt2 = 2 * (N / col_w);
-  constexpr size_t kData = (size_t)ROWS * 2 * N * sizeof(C);
-  const C* tw = stage_twiddles<tw_in_lds(kData, N, sizeof(C))>(tws, N, reinterpret_cast<C*>(smem) + ROWS * 2 * N);
-
-  // Real-space normalization (tensor_tools.py:301-305) from the column
-  // pass's k-space sums by Parseval: sum_r |IDFT2 v|^2 = Nx Ny sum_k |v|^2.
-  double nrm[2];
-  part_sums(P.part, nt2, PART_NRM, 0, nrm, smem);
-  if (first_sub && blockIdx.x == 0) {
-    double s[2];
-    part_sums(P.part, nt2, PART_S, 1, s, smem);
-    if (threadIdx.x == 0) {
-      const int step = run.step;
-      if (step > 0 && run.pops && step - 1 < run.pops_cap) {
-        const double tot = s[0] + s[1];
-        run.pops[2 * (step - 1) + 0] = P.atom_num * s[0] / tot;
-        run.pops[2 * (step - 1) + 1] = P.atom_num * s[1] / tot;
-      }
-      run.step = step + 1;
-    }
-  }
-  // The state leaves this pass in hybrid form (no other block of this kernel
-  // reads the flag; the next column pass does).
-  if (blockIdx.x == 0 && threadIdx.x == 0) run.form = FORM_HYBRID;
-  const double lam2 = P.atom_num / (P.dv_r * (double)N * (double)ny * nrm[0]);
-
   const int row = threadIdx.x / T;
   const int t = threadIdx.x % T;
   const int y = blockIdx.x * ROWS + row;
+  C* const sm = reinterpret_cast<C*>(smem);
   C* r0 = reinterpret_cast<C*>(P.psi) + (size_t)y * N;
   C* r1 = r0 + (size_t)ny * N;
-  C* lds0 = reinterpret_cast<C*>(smem) + (size_t)row * 2 * N;
+  C* lds0 = sm + (size_t)row * 2 * N;
   C* lds1 = lds0 + N;
 
+  // Global loads first: the two rows, the twiddles (held in registers until
+  // they are stored to LDS) and the norm partials; the kernel then waits for
+  // memory once.
   C a0[E], a1[E];
 #pragma unroll
   for (int j = 0; j < E; ++j) {
     a0[j] = r0[t + j * T];
     a1[j] = r1[t + j * T];
   }
-  fft_run2<N, E, true, V>(a0, a1, t, tw, lds0, lds1);
-
-  const bool imag = P.imag != 0;
-  const R lam = (R)sqrt(lam2);
-  const R lam2r = (R)lam2;
-  const R cf = (R)P.fft_c;
-  const R guu = (R)P.g_uu, gdd = (R)P.g_dd, gud = (R)P.g_ud;
-  const R hdt = (R)(0.5 * P.dt_sub[dur]);
-  const R dts = (R)P.dt_sub[dur];
+  constexpr int kBlock = ROWS * T;
   const double* vd0 = P.v_dense[0];
   const double* vd1 = P.v_dense[1];
   C py0 = C{(R)1, (R)0}, py1 = C{(R)1, (R)0};
@@ -280,10 +268,53 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
     py1 = reinterpret_cast<const C*>(P.py[dur][1])[y];
     px1 = reinterpret_cast<const C*>(P.px[dur][1]);
   }
+  const C* eth = reinterpret_cast<const C*>(P.etheta);
+  TableStage<kTw, N, kBlock, C> stw;
+  stw.load(tws, N - 1);
+
+  // Real-space normalization (tensor_tools.py:301-305) from the column
+  // pass's k-space sums by Parseval: sum_r |IDFT2 v|^2 = Nx Ny sum_k |v|^2.
+  // Its loads join the ones above; the reduction's barriers publish the
+  // twiddles stored just before it.
+  double pv[2] = {0.0, 0.0}, ps[2] = {0.0, 0.0};
+  const bool pops_block = first_sub && blockIdx.x == 0;
+  for (int i = threadIdx.x; i < nt2; i += blockDim.x) {
+    pv[0] += P.part[PART_NRM * nt2 + i];
+    if (pops_block) ps[i >= nt2 / 2 ? 1 : 0] += P.part[PART_S * nt2 + i];
+  }
+  const C* tw = stw.store(sm + ROWS * 2 * N, N - 1);
+  double nrm[2];
+  block_sum<2>(pv, smem);
+  nrm[0] = pv[0];
+  if (pops_block) {
+    block_sum<2>(ps, smem);
+    if (threadIdx.x == 0) {
+      const int step = run.step;
+      if (step > 0 && run.pops && step - 1 < run.pops_cap) {
+        const double tot = ps[0] + ps[1];
+        run.pops[2 * (step - 1) + 0] = P.atom_num * ps[0] / tot;
+        run.pops[2 * (step - 1) + 1] = P.atom_num * ps[1] / tot;
+      }
+      run.step = step + 1;
+    }
+  }
+  // The state leaves this pass in hybrid form (no other block of this kernel
+  // reads the flag; the next column pass does).
+  if (blockIdx.x == 0 && threadIdx.x == 0) run.form = FORM_HYBRID;
+  const double lam2 = P.atom_num / (P.dv_r * (double)N * (double)ny * nrm[0]);
+
+  fft_run2<N, E, true, V>(a0, a1, t, tw, lds0, lds1);
+
+  const bool imag = P.imag != 0;
+  const R lam = (R)sqrt(lam2);
+  const R lam2r = (R)lam2;
+  const R cf = (R)P.fft_c;
+  const R guu = (R)P.g_uu, gdd = (R)P.g_dd, gud = (R)P.g_ud;
+  const R hdt = (R)(0.5 * P.dt_sub[dur]);
+  const R dts = (R)P.dt_sub[dur];
   const int cpl = P.coupling;
   const R ca = (R)P.c_alpha[dur];
   const C cb = C{(R)P.c_beta_re[dur], (R)P.c_beta_im[dur]};
-  const C* eth = reinterpret_cast<const C*>(P.etheta);
   const double* omd = P.omega_dense;
 
 #pragma unroll
