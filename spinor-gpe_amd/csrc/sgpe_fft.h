@@ -220,36 +220,51 @@ __device__ __forceinline__ const C* stage_twiddles(const C* __restrict__ tw, int
   }
 }
 
+// Registers of one thread's share of a table: K entries, each held as two
+// scalar members (member access to scalars keeps them in VGPRs, where an
+// indexed array of vector-typed entries may be put in scratch).
+template <typename C> struct CxScalar;
+template <> struct CxScalar<double2> { typedef double T; };
+template <> struct CxScalar<float2> { typedef float T; };
+template <int K, typename C> struct TableRegs {
+  typename CxScalar<C>::T re, im;
+  TableRegs<K - 1, C> rest;
+  template <int N, int BLOCK> __device__ __forceinline__ void load(const C* __restrict__ s, int i) {
+    if (N % BLOCK == 0 || i < N) {
+      const C v = s[i];
+      re = v.x;
+      im = v.y;
+    }
+    rest.template load<N, BLOCK>(s, i + BLOCK);
+  }
+  template <int N, int BLOCK> __device__ __forceinline__ void store(C* dst, int i) const {
+    if (N % BLOCK == 0 || i < N) dst[i] = C{re, im};
+    rest.template store<N, BLOCK>(dst, i + BLOCK);
+  }
+};
+template <typename C> struct TableRegs<0, C> {
+  template <int N, int BLOCK> __device__ __forceinline__ void load(const C*, int) {}
+  template <int N, int BLOCK> __device__ __forceinline__ void store(C*, int) const {}
+};
 // Two-phase copy of an N-entry table into LDS by a block of BLOCK threads:
 // load() issues the global loads into registers, store() writes them to LDS
 // later, so that the loads of a kernel's prologue are all in flight together.
 // With IN_LDS false it only forwards the global pointer. The caller makes the
 // stores visible with a barrier before the first read.
 template <bool IN_LDS, int N, int BLOCK, typename C> struct TableStage {
-  static constexpr int kPer = IN_LDS ? (N + BLOCK - 1) / BLOCK : 1;
-  C v[kPer];
+  static constexpr int kPer = IN_LDS ? (N + BLOCK - 1) / BLOCK : 0;
+  TableRegs<kPer, C> regs;
   const C* src = nullptr;
-  __device__ __forceinline__ void load(const C* __restrict__ s, int count = N) {
+  __device__ __forceinline__ void load(const C* __restrict__ s) {
     src = s;
-    if constexpr (IN_LDS) {
-#pragma unroll
-      for (int q = 0; q < kPer; ++q) {
-        const int i = threadIdx.x + q * BLOCK;
-        if (i < count) v[q] = s[i];
-      }
-    }
+    regs.template load<N, BLOCK>(s, threadIdx.x);
   }
-  __device__ __forceinline__ const C* store(C* dst, int count = N) {
+  __device__ __forceinline__ const C* store(C* dst) {
     if constexpr (IN_LDS) {
-#pragma unroll
-      for (int q = 0; q < kPer; ++q) {
-        const int i = threadIdx.x + q * BLOCK;
-        if (i < count) dst[i] = v[q];
-      }
+      regs.template store<N, BLOCK>(dst, threadIdx.x);
       return dst;
     } else {
       (void)dst;
-      (void)count;
       return src;
     }
   }

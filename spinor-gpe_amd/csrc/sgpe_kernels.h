@@ -118,6 +118,16 @@ template <typename R> __device__ __forceinline__ double norm2d(typename Cx<R>::T
 
 // ------------------------------------------------------------ column pass
 
+// Workgroups are dealt to the 8 XCDs (each with its own L2) round-robin in
+// dispatch order. With n tiles a multiple of 8, XCD k gets the contiguous
+// tiles [k n/8, (k+1) n/8), so neighbouring column tiles, which share cache
+// lines when a tile row is narrower than a line (complex64, 8 columns),
+// meet in one L2 instead of being fetched twice.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  return n % kXcds == 0 ? (b % kXcds) * (n / kXcds) + b / kXcds : b;
+}
+
 template <typename R, int N, int E, int W>
 __global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __restrict__ insts,
                                                              RunDev* __restrict__ runs,
@@ -137,7 +147,7 @@ __global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __re
   const InstDev& P = insts[inst];
   const int nx = P.nx;
   const int comp = blockIdx.y;
-  const int tile = blockIdx.x;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int c = threadIdx.x % W;
   const int t = threadIdx.x / W;
   const int jx = tile * W + c;
@@ -158,13 +168,16 @@ __global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __re
   if (!td) {
     kxp = reinterpret_cast<const C*>(P.kx[d_prev][comp])[jx];
     kxc = reinterpret_cast<const C*>(P.kx[d_cur][comp])[jx];
-    sp.load(reinterpret_cast<const C*>(P.ky[d_prev][comp]));
-    sc_.load(reinterpret_cast<const C*>(P.ky[d_cur][comp]));
   }
-  stw.load(tws, N - 1);
-  const C* kyp = td ? nullptr : sp.store(sm + W * N + N);
-  const C* kyc = td ? nullptr : sc_.store(sm + W * N + 2 * N);
-  const C* tw = stw.store(sm + W * N, N - 1);
+  // The y-tables are staged unconditionally (a conditional fill puts the
+  // staging registers in scratch); without them (dense kinetic operator) the
+  // twiddle table, also N entries, stands in and the copies go unused.
+  sp.load(td ? tws : reinterpret_cast<const C*>(P.ky[d_prev][comp]));
+  sc_.load(td ? tws : reinterpret_cast<const C*>(P.ky[d_cur][comp]));
+  stw.load(tws);
+  const C* kyp = sp.store(sm + W * N + N);
+  const C* kyc = sc_.store(sm + W * N + 2 * N);
+  const C* tw = stw.store(sm + W * N);
 
   if (!kspace)
     fft_run<N, E, false, V>(a, t, tw, lds);
@@ -270,7 +283,7 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
   }
   const C* eth = reinterpret_cast<const C*>(P.etheta);
   TableStage<kTw, N, kBlock, C> stw;
-  stw.load(tws, N - 1);
+  stw.load(tws);
 
   // Real-space normalization (tensor_tools.py:301-305) from the column
   // pass's k-space sums by Parseval: sum_r |IDFT2 v|^2 = Nx Ny sum_k |v|^2.
@@ -282,7 +295,7 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
     pv[0] += P.part[PART_NRM * nt2 + i];
     if (pops_block) ps[i >= nt2 / 2 ? 1 : 0] += P.part[PART_S * nt2 + i];
   }
-  const C* tw = stw.store(sm + ROWS * 2 * N, N - 1);
+  const C* tw = stw.store(sm + ROWS * 2 * N);
   double nrm[2];
   block_sum<2>(pv, smem);
   nrm[0] = pv[0];
