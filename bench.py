@@ -13,7 +13,8 @@ Prints one JSON line. ``value`` is the complex128 rate (the reference's
 precision); the complex64 rate is reported under ``"c64"``. ``roofline``
 prices the dominant kernel's algorithmic bytes (read + write of the state,
 2 * 2 * Nx * Ny * 16 B) against the 8 TB/s HBM peak, with per-kernel average
-durations from HIP events on the launch stream. ``cpu_baseline`` times the
+durations from HIP events on the launch stream; ``traffic`` is that kernel's
+measured HBM bytes per dispatch (rocprofv3 counters, profiles folder). ``cpu_baseline`` times the
 CPU oracle (an op-for-op torch restatement of the reference's path) on a
 bounded sample of the same workload.
 """
@@ -35,6 +36,7 @@ for _p in (os.path.join(ROOT, 'spinor-gpe_amd'), ROOT):
 import torch  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = 'profiles/hbm_traffic_512x512.json'
 TITAN_V_512 = 149.1  # split-steps/s, bench_data/TitanV_cuda_step.npz (512^2, complex128)
 W50 = 2 * np.pi * 50
 WAVEL = 790.1e-9
@@ -113,6 +115,19 @@ def time_engine(ps, device, precision, steps, warmup, block, prof_steps, tune):
             'tuning': plan.tuning(), 'atoms': atoms}
 
 
+def measured_traffic(kernel):
+    """HBM bytes per dispatch of one kernel instance, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes summarized in the profiles folder (each
+    counter scaled by what it reads for a 256 MiB copy); None if that kernel
+    instance was not profiled."""
+    try:
+        with open(os.path.join(ROOT, TRAFFIC_FILE)) as f:
+            entry = json.load(f)['kernels'].get(kernel)
+    except (OSError, ValueError, KeyError):
+        return None
+    return entry['traffic_bytes'] if entry else None
+
+
 def cpu_baseline(ps, budget_s):
     """Time the CPU oracle's full_step() on the same workload (bounded sample)."""
     from oracle.reference_path import OraclePropagator
@@ -182,9 +197,15 @@ def main():
         ms = max(res['row_ms'], res['col_ms'])
         alg = 2 * 2 * n_pts * bytes_c[prec]
         achieved = alg / (ms * 1e-3) / 1e9
+        row_e, row_rows, col_e, col_w = res['tuning']
+        real = 'double' if prec == 'c128' else 'float'
+        kname = (f"row_pass_kernel<{real}, 512, {row_e}, {row_rows}>" if dom == 'row_pass'
+                 else f"col_pass_kernel<{real}, 512, {col_e}, {col_w}>")
+        traffic = measured_traffic(kname)
         return {'bound': 'hbm', 'kernel': dom, 'achieved': round(achieved, 1),
                 'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4),
-                'traffic': None, 'avg_us': round(ms * 1e3, 3), 'algorithmic_bytes': alg,
+                'traffic': traffic, 'traffic_source': TRAFFIC_FILE if traffic else None,
+                'kernel_instance': kname, 'avg_us': round(ms * 1e3, 3), 'algorithmic_bytes': alg,
                 'col_pass_us': round(res['col_ms'] * 1e3, 3), 'row_pass_us': round(res['row_ms'] * 1e3, 3)}
 
     r64 = results.get('c64')
