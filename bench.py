@@ -115,6 +115,65 @@ def time_engine(ps, device, precision, steps, warmup, block, prof_steps, tune):
             'tuning': plan.tuning(), 'atoms': atoms}
 
 
+def sweep_leg(device, check_every, fp64_sample):
+    """Batched ground-state sweep: 96 seeded Raman instances on 128^2, 256^2
+    and 512^2 grids solved together in complex64 (FP64 reductions) to a
+    relative energy change of 1e-8 per step or 8000 steps, against the same
+    set solved in complex128 (the accuracy reference) and against the plain
+    FP64 form: one instance at a time in complex128."""
+    from spinor_gpe.engine import workloads as wl
+    from spinor_gpe.engine.sweep import Sweep
+    draws = wl.sweep_draws(seed=2)
+    spins = [wl.raman_spinor(f'sweep{i}', d) for i, d in enumerate(draws)]
+    conv, cap, dt = 1e-8, 8000, 0.02
+
+    def solve(prec, items):
+        sw = Sweep(device, precision=prec, conv_tol=conv, check_every=check_every, max_steps=cap)
+        for ps in items:
+            sw.add_spin(ps, dt)
+        return sw.run(keep_states=False)
+
+    log("sweep: 96 instances, complex64 batched")
+    res32, wall32 = solve('c64', spins)
+    log(f"sweep: c64 batched {wall32:.2f} s; complex128 batched (accuracy reference)")
+    res64, wall64 = solve('c128', spins)
+    rel = np.array([abs(a.energy['E_tot'] - b.energy['E_tot']) / abs(b.energy['E_tot'])
+                    for a, b in zip(res32, res64)])
+    ok = np.array([a.converged and r <= 1e-6 for a, r in zip(res32, rel)])
+    single = {}
+    for grid in sorted({d.grid for d in draws}):
+        idx = [i for i, d in enumerate(draws) if d.grid == grid][:fp64_sample]
+        times = []
+        for i in idx:
+            _, w = solve('c128', [spins[i]])
+            times.append(w)
+        single[grid] = float(np.mean(times))
+        log(f"sweep: FP64 single-instance {grid}^2: {single[grid]:.3f} s per solve ({len(idx)} sampled)")
+    t_single = sum(single[d.grid] for d in draws)
+    rate32 = int(ok.sum()) / wall32
+    rate_single = len(draws) / t_single
+    steps = np.array([r.steps for r in res32])
+    return {
+        'workload': '96 Raman ground states (32x128^2, 48x256^2, 16x512^2), r=16, kL=3 rotated frame, '
+                    'Omega~U[0,4]E_R, delta~U[-1,1]E_R, N~10^U[3,5], g_ud/g_uu~U[0.9,1.1], seed 2, '
+                    'imaginary dt=0.02',
+        'precision': 'c64 (f32 arithmetic, f64 reductions)', 'conv_tol_per_step': conv,
+        'max_steps': cap, 'check_every': check_every,
+        'wall_s': round(wall32, 3), 'converged': int(sum(r.converged for r in res32)),
+        'solves_within_1e-6': int(ok.sum()), 'solves_per_s': round(rate32, 3),
+        'steps_mean': float(steps.mean()), 'steps_max': int(steps.max()),
+        'energy_rel_err_vs_c128': {'max': float(rel.max()), 'median': float(np.median(rel))},
+        'c128_batched': {'wall_s': round(wall64, 3), 'converged': int(sum(r.converged for r in res64)),
+                         'solves_per_s': round(len(draws) / wall64, 3)},
+        'fp64_single': {'sampled_per_grid': fp64_sample,
+                        'solve_s_by_grid': {str(g): round(t, 4) for g, t in single.items()},
+                        'solves_per_s': round(rate_single, 3),
+                        'note': 'one instance at a time, complex128; per-grid mean solve time of the '
+                                'sampled instances applied to the whole set'},
+        'ratio_vs_fp64_single': round(rate32 / rate_single, 2),
+    }
+
+
 def measured_traffic(kernel):
     """HBM bytes per dispatch of one kernel instance, from the rocprofv3
     FETCH_SIZE / WRITE_SIZE passes summarized in the profiles folder (each
@@ -146,40 +205,8 @@ def cpu_baseline(ps, budget_s):
                        f"of the torch-CPU oracle (median {med * 1e3:.1f} ms)")}
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__.split('\n')[0])
-    ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=20000)
-    ap.add_argument('--warmup', type=int, default=300)
-    ap.add_argument('--block', type=int, default=100, help='full steps per captured graph')
-    ap.add_argument('--profile-steps', type=int, default=200)
-    ap.add_argument('--cpu-budget', type=float, default=15.0, help='seconds of CPU baseline sampling')
-    ap.add_argument('--no-cpu-baseline', action='store_true')
-    ap.add_argument('--tune', action='store_true',
-                    help='re-measure the FFT shapes of the workload instead of using the shipped table')
-    ap.add_argument('--precisions', nargs='+', default=['c128', 'c64'], choices=['c128', 'c64'])
-    ap.add_argument('--calibrate-copy', type=int, default=0, metavar='MIB',
-                    help='before timing, copy a MIB-sized device buffer once (a known-size '
-                         'copy for scaling HBM counters in profiles)')
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this engine measures one GPU; --gpus {args.gpus} is not supported "
-              "(instances shard across GPUs without communication)", file=sys.stderr)
-        sys.exit(2)
-    if not torch.cuda.is_available():
-        print("bench.py: no HIP device available", file=sys.stderr)
-        sys.exit(1)
-    os.environ.setdefault('SPINOR_GPE_DATA', tempfile.mkdtemp(prefix='sgpe_bench_'))
-    if args.tune:
-        os.environ['SGPE_TUNE_FILE'] = os.path.join(tempfile.mkdtemp(prefix='sgpe_tune_'), 'tune.json')
-    device = 'cuda:0'
-    if args.calibrate_copy:
-        src = torch.ones(args.calibrate_copy * 2 ** 20 // 4, dtype=torch.float32, device=device)
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"calibration copy: {args.calibrate_copy} MiB read + {args.calibrate_copy} MiB written")
-        del src, dst
+def step_leg(args, device):
+    """The metric value: 512^2 split-steps/s in complex128 (and complex64)."""
     log("building the 512^2 initial state (256^2 ground state, 4000 imaginary steps)")
     ps = c1_spinor(device)
     n_pts = 512 * 512
@@ -234,6 +261,51 @@ def main():
     if not args.no_cpu_baseline:
         log(f"CPU baseline: oracle full_step() for ~{args.cpu_budget:.0f} s")
         line['cpu_baseline'] = cpu_baseline(ps, args.cpu_budget)
+    return line
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split('\n')[0])
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20000)
+    ap.add_argument('--warmup', type=int, default=300)
+    ap.add_argument('--block', type=int, default=100, help='full steps per captured graph')
+    ap.add_argument('--profile-steps', type=int, default=200)
+    ap.add_argument('--cpu-budget', type=float, default=15.0, help='seconds of CPU baseline sampling')
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--tune', action='store_true',
+                    help='re-measure the FFT shapes of the workload instead of using the shipped table')
+    ap.add_argument('--precisions', nargs='+', default=['c128', 'c64'], choices=['c128', 'c64'])
+    ap.add_argument('--legs', nargs='+', default=['step', 'sweep'], choices=['step', 'sweep'],
+                    help='step: 512^2 split-step rate (the metric value); sweep: batched ground states')
+    ap.add_argument('--sweep-check', type=int, default=100, help='full steps between convergence checks')
+    ap.add_argument('--fp64-sample', type=int, default=4,
+                    help='instances per grid timed one at a time in complex128')
+    ap.add_argument('--calibrate-copy', type=int, default=0, metavar='MIB',
+                    help='before timing, copy a MIB-sized device buffer once (a known-size '
+                         'copy for scaling HBM counters in profiles)')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this engine measures one GPU; --gpus {args.gpus} is not supported "
+              "(instances shard across GPUs without communication)", file=sys.stderr)
+        sys.exit(2)
+    if not torch.cuda.is_available():
+        print("bench.py: no HIP device available", file=sys.stderr)
+        sys.exit(1)
+    os.environ.setdefault('SPINOR_GPE_DATA', tempfile.mkdtemp(prefix='sgpe_bench_'))
+    if args.tune:
+        os.environ['SGPE_TUNE_FILE'] = os.path.join(tempfile.mkdtemp(prefix='sgpe_tune_'), 'tune.json')
+    device = 'cuda:0'
+    if args.calibrate_copy:
+        src = torch.ones(args.calibrate_copy * 2 ** 20 // 4, dtype=torch.float32, device=device)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calibrate_copy} MiB read + {args.calibrate_copy} MiB written")
+        del src, dst
+    line = step_leg(args, device) if 'step' in args.legs else {'metric': METRIC, 'value': None}
+    if 'sweep' in args.legs:
+        line['sweep'] = sweep_leg(device, args.sweep_check, args.fp64_sample)
     print(json.dumps(line), flush=True)
 
 
