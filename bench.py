@@ -174,6 +174,43 @@ def sweep_leg(device, check_every, fp64_sample):
     }
 
 
+def serve_leg(device, n_requests, load):
+    """Served stream: seeded requests on 128^2-1024^2 grids (p = .3/.4/.2/.1),
+    70 % ground states to convergence, 30 % quenches (ground state, then
+    delta += 0.5 E_R and 5000 real-time steps of dt = 1e-3), complex64 with
+    FP64 reductions. A burst (all requests at t = 0) measures the capacity;
+    then the same requests arrive as a Poisson stream at ``load`` x capacity
+    and their arrival-to-result latencies are reported."""
+    from spinor_gpe.engine import serve
+    srv = serve.Server(device, 'c64')
+    log("serve: warm-up (one request per grid size)")
+    warm = [serve.Request(1000 + i, r.draw, 'ground', 0.0)
+            for i, r in enumerate(serve.request_stream(4, None, seed=99, grids=((128, 1), (256, 1), (512, 1), (1024, 1))))]
+    for i, g in enumerate((128, 256, 512, 1024)):
+        warm[i].draw.grid = g
+    srv.run(warm)
+    reqs = serve.request_stream(n_requests, None, seed=3)
+    log(f"serve: burst of {n_requests} requests")
+    burst = srv.run(reqs)
+    makespan = max(o.finished for o in burst)
+    capacity = n_requests / makespan
+    rate = load * capacity
+    log(f"serve: capacity {capacity:.2f} solves/s; Poisson stream at {rate:.2f}/s")
+    stream = serve.request_stream(n_requests, rate, seed=3)
+    out = srv.run(stream)
+    wall = max(o.finished for o in out)
+    res = serve.summarize(out, wall)
+    res.update({
+        'workload': f'{n_requests} requests, grids 128/256/512/1024 at p=.3/.4/.2/.1, 70% ground state '
+                    '(imaginary dt=0.02 to |dE/E|<=1e-8 per step or 8000 steps), 30% quench (+0.5 E_R '
+                    'detuning, 5000 real-time steps dt=1e-3), Raman kL=3 rotated frame, seed 3',
+        'precision': 'c64 (f32 arithmetic, f64 reductions)',
+        'slots': {str(k): v for k, v in srv.slots.items()}, 'block_steps': srv.block,
+        'capacity_solves_per_s': round(capacity, 3), 'arrival_rate_per_s': round(rate, 3),
+        'load': load, 'burst': serve.summarize(burst, makespan)})
+    return res
+
+
 def measured_traffic(kernel):
     """HBM bytes per dispatch of one kernel instance, from the rocprofv3
     FETCH_SIZE / WRITE_SIZE passes summarized in the profiles folder (each
@@ -276,8 +313,12 @@ def main():
     ap.add_argument('--tune', action='store_true',
                     help='re-measure the FFT shapes of the workload instead of using the shipped table')
     ap.add_argument('--precisions', nargs='+', default=['c128', 'c64'], choices=['c128', 'c64'])
-    ap.add_argument('--legs', nargs='+', default=['step', 'sweep'], choices=['step', 'sweep'],
-                    help='step: 512^2 split-step rate (the metric value); sweep: batched ground states')
+    ap.add_argument('--legs', nargs='+', default=['step', 'sweep', 'serve'],
+                    choices=['step', 'sweep', 'serve'],
+                    help='step: 512^2 split-step rate (the metric value); sweep: batched ground '
+                         'states; serve: continuous-batching request stream')
+    ap.add_argument('--requests', type=int, default=48, help='requests of the served stream')
+    ap.add_argument('--load', type=float, default=0.8, help='arrival rate / measured capacity')
     ap.add_argument('--sweep-check', type=int, default=100, help='full steps between convergence checks')
     ap.add_argument('--fp64-sample', type=int, default=4,
                     help='instances per grid timed one at a time in complex128')
@@ -306,6 +347,8 @@ def main():
     line = step_leg(args, device) if 'step' in args.legs else {'metric': METRIC, 'value': None}
     if 'sweep' in args.legs:
         line['sweep'] = sweep_leg(device, args.sweep_check, args.fp64_sample)
+    if 'serve' in args.legs:
+        line['serve'] = serve_leg(device, args.requests, args.load)
     print(json.dumps(line), flush=True)
 
 

@@ -119,7 +119,9 @@ int sgpe_pops(sgpe_ctx* ctx, int precision, int ny, int nx, const void* psi0, co
 
 /* A plan of n_slots instances sharing one grid (ny, nx) and precision.
  * Stands in for TensorPropagator's operator set-up (tensor_propagator.py:
- * 67-149), batched. ny, nx: powers of two in [64, 4096]. */
+ * 67-149), batched. ny, nx: powers of two in [64, 4096]. Slots start empty
+ * and inactive; a plan can step once at least one slot is set (empty slots
+ * are skipped), so a server can fill slots as requests arrive. */
 int sgpe_plan_create(sgpe_ctx* ctx, int precision, int ny, int nx, int n_slots, sgpe_plan** out);
 void sgpe_plan_destroy(sgpe_plan* plan);
 
@@ -132,7 +134,7 @@ int sgpe_plan_set_slot(sgpe_plan* plan, int slot, const sgpe_inst_desc* desc, vo
  * run; physics tables are kept. Synchronizes the stream. */
 int sgpe_plan_set_state(sgpe_plan* plan, int slot, void* psik);
 
-/* Start a run for one slot (and activate it): device buffer pops (2 doubles
+/* Start a run for one (set) slot and activate it: device buffer pops (2 doubles
  * per full step, capacity cap steps; NULL for none) receives the per-step
  * populations of prop_loop (tensor_propagator.py:173-194); conv_tol > 0
  * enables the convergence test of sgpe_energy, which deactivates the slot

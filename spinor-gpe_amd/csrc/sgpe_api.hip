@@ -553,11 +553,14 @@ static int plan_load_slot(sgpe_plan* p, int slot, const sgpe_inst_desc* d, void*
   return SGPE_OK;
 }
 
+// A plan steps once any slot is set; slots never set keep active = 0 (the
+// run state starts zeroed and only a set slot can be activated), so no
+// kernel reads their empty descriptors.
 static int plan_ready(const sgpe_plan* p) {
   if (!p) return fail(SGPE_E_ARG, "plan is NULL");
   for (int i = 0; i < p->nslots; ++i)
-    if (!p->slots[i].loaded) return fail(SGPE_E_ARG, "slot " + std::to_string(i) + " has not been set");
-  return SGPE_OK;
+    if (p->slots[i].loaded) return SGPE_OK;
+  return fail(SGPE_E_ARG, "no slot of the plan has been set");
 }
 
 // ------------------------------------------------------------------- C ABI
@@ -783,6 +786,7 @@ int sgpe_plan_set_state(sgpe_plan* p, int slot, void* psik) {
 
 int sgpe_plan_set_run(sgpe_plan* p, int slot, double* pops, int cap, double conv_tol) {
   if (!p || slot < 0 || slot >= p->nslots || cap < 0) return fail(SGPE_E_ARG, "bad arguments");
+  if (!p->slots[slot].loaded) return fail(SGPE_E_ARG, "slot has not been set");
   HIP_TRY(hipSetDevice(p->ctx->device));
   hipLaunchKernelGGL(config_run_kernel, dim3(1), dim3(64), 0, p->ctx->stream, p->d_run, slot, pops, pops ? cap : 0, 1,
                      1, -1, conv_tol);
