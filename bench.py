@@ -38,8 +38,6 @@ import torch  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = 'profiles/hbm_traffic_512x512.json'
 TITAN_V_512 = 149.1  # split-steps/s, bench_data/TitanV_cuda_step.npz (512^2, complex128)
-W50 = 2 * np.pi * 50
-WAVEL = 790.1e-9
 METRIC = ("split-steps/s at 512² (1 GPU); sweep solves/s + p50/p99 latency at "
           "ΔE≤1e-6")
 
@@ -48,47 +46,13 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def ground_state_256(device):
-    """Imaginary-time ground state of the 256^2 / +/-16 configuration."""
-    from spinor_gpe.pspinor import pspinor as spin
-    ps = spin.PSpinor('c0', omeg={'x': W50, 'y': W50, 'z': 40 * W50},
-                      g_sc={'uu': 1.0, 'dd': 1.0, 'ud': 0.995}, mesh_points=(256, 256),
-                      r_sizes=(16, 16), atom_num=1e4, pop_frac=(0.5, 0.5))
-    rng = np.random.default_rng(0)
-    ph = rng.uniform(-0.1, 0.1, (2, 256, 256))
-    ps.psi = [p * np.exp(1j * f) for p, f in zip(ps.psi, ph)]
-    c = ps.space['dv_r'] / (2 * np.pi)
-    ps.psik = [np.fft.fftshift(np.fft.fftn(p) * c) for p in ps.psi]
-    res, _ = ps.imaginary(0.02, 4000, device, progress=False)
-    return ps, res.psi
-
-
 def c1_spinor(device):
-    """The 512^2 Raman-coupled instance, initial state as described above."""
-    from scipy.interpolate import RegularGridInterpolator
-    from spinor_gpe.pspinor import pspinor as spin
-    ps0, psi0 = ground_state_256(device)
-    ps = spin.PSpinor('c1', omeg={'x': W50, 'y': W50, 'z': 40 * W50},
-                      g_sc={'uu': 1.0, 'dd': 1.0, 'ud': 0.995}, mesh_points=(512, 512),
-                      r_sizes=(24, 24), atom_num=1e4, pop_frac=(0.5, 0.5))
-    scale = 3.0 / (np.sqrt(2) * np.pi / WAVEL * ps.a_x)
-    ps.coupling_setup(wavel=WAVEL, scale=scale, kin_shift=False)
-    ps.coupling_uniform(2.0 * ps.EL_recoil)
-    ps.detuning_uniform(0.5 * ps.EL_recoil)
-    ps.rot_coupling = False
-    pts = np.stack([ps.space['y_mesh'].ravel(), ps.space['x_mesh'].ravel()], axis=1)
-    rng = np.random.default_rng(1)
-    psi = []
-    for p in psi0:
-        f = [RegularGridInterpolator((ps0.space['y'], ps0.space['x']), part, bounds_error=False,
-                                     fill_value=0.0) for part in (p.real, p.imag)]
-        v = (f[0](pts) + 1j * f[1](pts)).reshape(512, 512)
-        psi.append(v * np.exp(1j * rng.normal(0.0, 0.05, (512, 512))))
-    tot = sum(np.sum(np.abs(p) ** 2) for p in psi) * ps.space['dv_r']
-    ps.psi = [p * np.sqrt(ps.atom_num / tot) for p in psi]
-    c = ps.space['dv_r'] / (2 * np.pi)
-    ps.psik = [np.fft.fftshift(np.fft.fftn(p) * c) for p in ps.psi]
-    return ps
+    """The 512^2 Raman-coupled instance; its initial state comes from the
+    256^2 imaginary-time ground state (4000 steps of dt = 0.02)."""
+    from spinor_gpe.engine import workloads as wl
+    ps0 = wl.c0_spinor('c0')
+    res, _ = ps0.imaginary(0.02, 4000, device, progress=False)
+    return wl.c1_spinor(res.psi, ps0.space, 'c1')
 
 
 def time_engine(ps, device, precision, steps, warmup, block, prof_steps, tune):
@@ -131,12 +95,13 @@ def sweep_leg(device, check_every, fp64_sample):
         sw = Sweep(device, precision=prec, conv_tol=conv, check_every=check_every, max_steps=cap)
         for ps in items:
             sw.add_spin(ps, dt)
-        return sw.run(keep_states=False)
+        res, wall = sw.run(keep_states=False)
+        return res, wall, sw.trace
 
     log("sweep: 96 instances, complex64 batched")
-    res32, wall32 = solve('c64', spins)
+    res32, wall32, trace32 = solve('c64', spins)
     log(f"sweep: c64 batched {wall32:.2f} s; complex128 batched (accuracy reference)")
-    res64, wall64 = solve('c128', spins)
+    res64, wall64, _ = solve('c128', spins)
     rel = np.array([abs(a.energy['E_tot'] - b.energy['E_tot']) / abs(b.energy['E_tot'])
                     for a, b in zip(res32, res64)])
     ok = np.array([a.converged and r <= 1e-6 for a, r in zip(res32, rel)])
@@ -145,7 +110,7 @@ def sweep_leg(device, check_every, fp64_sample):
         idx = [i for i, d in enumerate(draws) if d.grid == grid][:fp64_sample]
         times = []
         for i in idx:
-            _, w = solve('c128', [spins[i]])
+            _, w, _ = solve('c128', [spins[i]])
             times.append(w)
         single[grid] = float(np.mean(times))
         log(f"sweep: FP64 single-instance {grid}^2: {single[grid]:.3f} s per solve ({len(idx)} sampled)")
@@ -162,6 +127,7 @@ def sweep_leg(device, check_every, fp64_sample):
         'wall_s': round(wall32, 3), 'converged': int(sum(r.converged for r in res32)),
         'solves_within_1e-6': int(ok.sum()), 'solves_per_s': round(rate32, 3),
         'steps_mean': float(steps.mean()), 'steps_max': int(steps.max()),
+        'active_trace': [[round(t, 3), n, a] for t, n, a in trace32],
         'energy_rel_err_vs_c128': {'max': float(rel.max()), 'median': float(np.median(rel))},
         'c128_batched': {'wall_s': round(wall64, 3), 'converged': int(sum(r.converged for r in res64)),
                          'solves_per_s': round(len(draws) / wall64, 3)},

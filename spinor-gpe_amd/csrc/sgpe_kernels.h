@@ -330,6 +330,22 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
   const C cb = C{(R)P.c_beta_re[dur], (R)P.c_beta_im[dur]};
   const double* omd = P.omega_dense;
 
+  // The x-table values of this thread's E points are loaded together, so
+  // the phase waits for memory once rather than once per point. An absent
+  // table is stood in for by the twiddles (N entries; values unused).
+  C tx0[E], tx1[E], tex[E];
+  {
+    const C* s0 = px0 ? px0 : tw;
+    const C* s1 = px1 ? px1 : tw;
+    const C* se = eth ? eth : tw;
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      tx0[j] = s0[t + j * T];
+      tx1[j] = s1[t + j * T];
+      tex[j] = se[t + j * T];
+    }
+  }
+
 #pragma unroll
   for (int j = 0; j < E; ++j) {
     const int x = t + j * T;
@@ -360,7 +376,7 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
         b10 = b01;
       }
       if (eth) {
-        const C e = eth[x];
+        const C e = tex[j];
         b10 = cmul(b01, cconj(e));
         b01 = cmul(b01, e);
       }
@@ -370,8 +386,8 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
       p1 = q1;
     }
     // Potential step P_s = exp(-i V_s Delta) (tensor_propagator.py:256).
-    const C q0 = vd0 ? phase_op<R>((R)vd0[(size_t)y * N + x] * dts, imag) : cmul(px0[x], py0);
-    const C q1 = vd1 ? phase_op<R>((R)vd1[(size_t)y * N + x] * dts, imag) : cmul(px1[x], py1);
+    const C q0 = vd0 ? phase_op<R>((R)vd0[(size_t)y * N + x] * dts, imag) : cmul(tx0[j], py0);
+    const C q1 = vd1 ? phase_op<R>((R)vd1[(size_t)y * N + x] * dts, imag) : cmul(tx1[j], py1);
     p0 = cmul(p0, q0);
     p1 = cmul(p1, q1);
     if (cpl) {

@@ -80,6 +80,7 @@ class Sweep:
         self.max_steps = int(max_steps)
         self.poll_every = int(poll_every)
         self.items = []
+        self.trace = []  # (seconds, full steps, active instances) at each poll
 
     def add(self, spec, psik):
         """Queue an instance (:class:`InstanceSpec`, k-space state (2, ny, nx))."""
@@ -109,17 +110,25 @@ class Sweep:
                 slots[i] = (len(plans), slot)
             plans.append(plan)
             states.append(st)
+        live = list(range(len(plans)))
         group = Group(plans)
         group.run(self.check_every, True, 0)  # allocations and sync outside the timed loop
         torch.cuda.synchronize(self.device)
         t0 = time.perf_counter()
         steps = 0
+        self.trace = []
         while steps < self.max_steps:
             n = min(self.poll_every, -(-(self.max_steps - steps) // self.check_every))
             group.run(self.check_every, True, n)
             steps += n * self.check_every
-            if all(not p.status()[0][:, 0].any() for p in plans):
+            active = {g: int(plans[g].status()[0][:, 0].sum()) for g in live}
+            self.trace.append((time.perf_counter() - t0, steps, sum(active.values())))
+            if not any(active.values()):
                 break
+            # plans whose instances have all converged leave the graph
+            if not all(active.values()):
+                live = [g for g in live if active[g]]
+                group = Group([plans[g] for g in live])
         torch.cuda.synchronize(self.device)
         wall = time.perf_counter() - t0
         results = []

@@ -315,14 +315,15 @@ class PSpinor:
         """Uniform detuning (pspinor.py:662-678)."""
         self.detuning = np.ones_like(self.space['x_mesh']) * value
 
-    def seed_vortices(self, positions, windings):
+    def seed_vortices(self, positions, windings, *, components=(0, 1)):
         """Imprint vortices on the real-space state (pspinor.py:680-745).
 
         Each vortex multiplies component ``i`` by ``r/sqrt(r^2 + xi_i^2)``
         times ``exp(+/- i atan2(dy, dx))``, with ``xi_i`` the component's
         healing length. ``positions`` is ``(n, 2)`` (shared by both
         components) or ``(2, n, 2)``; ``windings`` is one value, one per
-        position, or ``(2, n)``.
+        position, or ``(2, n)``. Keyword-only addition: ``components`` lists
+        the components to imprint (default both, as in the reference).
         """
         positions = np.array(positions)
         assert positions.shape[-1] == 2, (
@@ -346,7 +347,7 @@ class PSpinor:
             assert windings.shape == positions.shape[:-1], (
                 "The number of supplied windings must match the number of "
                 "supplied positions")
-        for i in range(len(self.psi)):
+        for i in components:
             for j, (x0, y0) in enumerate(positions[i]):
                 dx = self.space['x_mesh'] - x0
                 dy = self.space['y_mesh'] - y0
