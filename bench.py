@@ -76,7 +76,7 @@ def time_engine(ps, device, precision, steps, warmup, block, prof_steps, tune):
     st = plan.states[0]
     atoms = float(torch.sum(torch.abs(st) ** 2).item()) * ps.space['dv_k']
     return {'elapsed': elapsed, 'col_ms': col_ms / n_col, 'row_ms': row_ms / n_row,
-            'tuning': plan.tuning(), 'atoms': atoms}
+            'tuning': plan.tuning(), 'streaming': plan.streaming(), 'atoms': atoms}
 
 
 def sweep_leg(device, check_every, fp64_sample):
@@ -231,6 +231,8 @@ def step_leg(args, device):
         real = 'double' if prec == 'c128' else 'float'
         kname = (f"row_pass_kernel<{real}, 512, {row_e}, {row_rows}>" if dom == 'row_pass'
                  else f"col_pass_kernel<{real}, 512, {col_e}, {col_w}>")
+        if res['streaming'][0 if dom == 'row_pass' else 1]:
+            kname += ' streaming'
         traffic = measured_traffic(kname)
         return {'bound': 'hbm', 'kernel': dom, 'achieved': round(achieved, 1),
                 'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4),
@@ -250,6 +252,7 @@ def step_leg(args, device):
                    'grid': [512, 512], 'r_sizes': [24, 24], 'dt': 1e-3, 'omega_ER': 2.0,
                    'delta_ER': 0.5, 'kL': 3.0, 'frame': 'lab', 'atoms': 1e4,
                    'graph_block_steps': args.block, 'tuning_row_e_rows_col_e_w': list(r['tuning']),
+                   'streaming_row_col': list(r['streaming']),
                    'baseline': 'Titan V 149.1 split-steps/s at 512^2, complex128 '
                                '(reference bench_data/TitanV_cuda_step.npz)'},
         'grid_point_steps_per_s': round(sps * n_pts, 1),

@@ -163,6 +163,12 @@ int sgpe_list_shapes(int pass, int precision, int n, int* out_pairs, int cap);
 /* Current shapes: out4 = {row_e, row_rows, col_e, col_w}. */
 int sgpe_plan_get_tuning(const sgpe_plan* plan, int* out4);
 
+/* Cache policy of the two passes' state reads and writes: 1 = non-temporal
+ * (streaming) accesses, 0 = normal. Default: streaming for single-slot plans,
+ * normal for batched ones. No reference counterpart (launch configuration). */
+int sgpe_plan_set_streaming(sgpe_plan* plan, int row_stream, int col_stream);
+int sgpe_plan_get_streaming(const sgpe_plan* plan, int* out2);
+
 /* single_step (tensor_propagator.py:224-271) for every active slot with
  * the outer (SGPE_SUB_OUTER) or inner (SGPE_SUB_INNER) sub-step duration and
  * operators. The closing kinetic half-step and renormalization are applied

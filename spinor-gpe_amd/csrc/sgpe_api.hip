@@ -201,6 +201,7 @@ struct sgpe_plan {
   sgpe_ctx* ctx = nullptr;
   int prec = 0, ny = 0, nx = 0, nslots = 0;
   int row_e = 8, row_rows = 1, col_e = 8, col_w = 4;
+  int row_stream = 0, col_stream = 0;  // non-temporal state accesses per pass
   const KEntry* row = nullptr;
   const KEntry* col = nullptr;
   const void* tw_x = nullptr;  // twiddles for nx (rows)
@@ -259,14 +260,16 @@ static void launch_col(sgpe_plan* p, hipStream_t s, int mode, int dprev, int dcu
                        hipEvent_t e1 = nullptr) {
   const size_t lds = lds_bytes((size_t)p->col_w * p->ny * csize(p->prec), p->ny, csize(p->prec), kColTables);
   dim3 grid(p->nx / p->col_w, 2, p->nslots), block(p->col_w * (p->ny / p->col_e));
-  ((ColFn)p->col->launch)(grid, block, lds, s, e0, e1, p->d_inst, p->d_run, p->tw_y, mode, dprev, dcur);
+  ((ColFn)p->col->launch)(grid, block, lds, s, e0, e1, p->d_inst, p->d_run, p->tw_y, mode, dprev, dcur,
+                          p->col_stream);
 }
 
 static void launch_row(sgpe_plan* p, hipStream_t s, int dur, int first, hipEvent_t e0 = nullptr,
                        hipEvent_t e1 = nullptr) {
   const size_t lds = lds_bytes((size_t)p->row_rows * 2 * p->nx * csize(p->prec), p->nx, csize(p->prec));
   dim3 grid(p->ny / p->row_rows, 1, p->nslots), block(p->row_rows * (p->nx / p->row_e));
-  ((RowFn)p->row->launch)(grid, block, lds, s, e0, e1, p->d_inst, p->d_run, p->tw_x, dur, first, p->col_w);
+  ((RowFn)p->row->launch)(grid, block, lds, s, e0, e1, p->d_inst, p->d_run, p->tw_x, dur, first, p->col_w,
+                          p->row_stream);
 }
 
 static void issue_sub_step(sgpe_plan* p, hipStream_t s, int which, int first) {
@@ -728,6 +731,7 @@ int sgpe_plan_create(sgpe_ctx* ctx, int precision, int ny, int nx, int n_slots, 
   p->ny = ny;
   p->nx = nx;
   p->nslots = n_slots;
+  p->row_stream = p->col_stream = n_slots == 1 ? 1 : 0;
   p->h_inst.resize(n_slots);
   p->slots.resize(n_slots);
   int re, rr, ce, cw;
@@ -841,6 +845,21 @@ int sgpe_plan_tune(sgpe_plan* p, int row_e, int row_rows, int col_e, int col_w) 
   HIP_TRY(hipSetDevice(p->ctx->device));
   return plan_select(p, row_e ? row_e : p->row_e, row_rows ? row_rows : p->row_rows, col_e ? col_e : p->col_e,
                      col_w ? col_w : p->col_w);
+}
+
+int sgpe_plan_set_streaming(sgpe_plan* p, int row_stream, int col_stream) {
+  if (!p) return fail(SGPE_E_ARG, "plan is NULL");
+  p->row_stream = row_stream ? 1 : 0;
+  p->col_stream = col_stream ? 1 : 0;
+  drop_graphs(p);
+  return SGPE_OK;
+}
+
+int sgpe_plan_get_streaming(const sgpe_plan* p, int* out2) {
+  if (!p || !out2) return fail(SGPE_E_ARG, "bad arguments");
+  out2[0] = p->row_stream;
+  out2[1] = p->col_stream;
+  return SGPE_OK;
 }
 
 int sgpe_plan_get_tuning(const sgpe_plan* p, int* out4) {

@@ -111,6 +111,24 @@ __device__ __forceinline__ typename Cx<R>::T phase_op(R e_dt, bool imag) {
   return C{c, s};
 }
 
+// State accesses of the split-step passes with a non-temporal hint: every
+// pass reads and writes each state element exactly once, and streaming them
+// past the caches shortens a pass when a single instance leaves the chip
+// mostly idle (for batched plans the normal policy can be faster; the plan
+// chooses, see the streaming flags of sgpe_plan_set_streaming).
+template <typename C> __device__ __forceinline__ C ld_stream(const C* p) {
+  typedef decltype(p->x) S;
+  typedef S V2 __attribute__((ext_vector_type(2)));
+  const V2 w = __builtin_nontemporal_load(reinterpret_cast<const V2*>(p));
+  return C{w.x, w.y};
+}
+template <typename C> __device__ __forceinline__ void st_stream(C* p, C v) {
+  typedef decltype(v.x) S;
+  typedef S V2 __attribute__((ext_vector_type(2)));
+  const V2 w = {v.x, v.y};
+  __builtin_nontemporal_store(w, reinterpret_cast<V2*>(p));
+}
+
 template <typename R> __device__ __forceinline__ double norm2d(typename Cx<R>::T a) {
   double x = (double)a.x, y = (double)a.y;
   return x * x + y * y;
@@ -132,7 +150,7 @@ template <typename R, int N, int E, int W>
 __global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __restrict__ insts,
                                                              RunDev* __restrict__ runs,
                                                              const typename Cx<R>::T* __restrict__ tws,
-                                                             int mode, int d_prev, int d_cur) {
+                                                             int mode, int d_prev, int d_cur, int stream) {
   typedef typename Cx<R>::T C;
   typedef LdsView<W, 1 << 30, 0> V;
   constexpr int T = N / E;
@@ -159,8 +177,13 @@ __global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __re
   // then the kinetic tables and twiddles (staged into LDS).
   constexpr int kBlock = W * T;
   C a[E];
+  if (stream) {
 #pragma unroll
-  for (int j = 0; j < E; ++j) a[j] = col[(size_t)(t + j * T) * nx];
+    for (int j = 0; j < E; ++j) a[j] = ld_stream(col + (size_t)(t + j * T) * nx);
+  } else {
+#pragma unroll
+    for (int j = 0; j < E; ++j) a[j] = col[(size_t)(t + j * T) * nx];
+  }
   const double* td = P.t_dense[comp];
   C kxp = C{(R)1, (R)0}, kxc = C{(R)1, (R)0};
   TableStage<kKy, N, kBlock, C> sp, sc_;
@@ -204,8 +227,13 @@ __global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __re
     }
   }
   if (mode == COL_MID) fft_run<N, E, true, V>(a, t, tw, lds);
+  if (stream) {
 #pragma unroll
-  for (int j = 0; j < E; ++j) col[(size_t)(t + j * T) * nx] = a[j];
+    for (int j = 0; j < E; ++j) st_stream(col + (size_t)(t + j * T) * nx, a[j]);
+  } else {
+#pragma unroll
+    for (int j = 0; j < E; ++j) col[(size_t)(t + j * T) * nx] = a[j];
+  }
 
   double v[2] = {s_acc, n_acc};
   block_sum<2>(v, smem);
@@ -234,7 +262,7 @@ template <typename R, int N, int E, int ROWS>
 __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* __restrict__ insts,
                                                                 RunDev* __restrict__ runs,
                                                                 const typename Cx<R>::T* __restrict__ tws,
-                                                                int dur, int first_sub, int col_w) {
+                                                                int dur, int first_sub, int col_w, int stream) {
   typedef typename Cx<R>::T C;
   typedef LdsView<1, 1 << 30, 0> V;
   constexpr int T = N / E;
@@ -262,10 +290,18 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
   // they are stored to LDS) and the norm partials; the kernel then waits for
   // memory once.
   C a0[E], a1[E];
+  if (stream) {
 #pragma unroll
-  for (int j = 0; j < E; ++j) {
-    a0[j] = r0[t + j * T];
-    a1[j] = r1[t + j * T];
+    for (int j = 0; j < E; ++j) {
+      a0[j] = ld_stream(r0 + t + j * T);
+      a1[j] = ld_stream(r1 + t + j * T);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      a0[j] = r0[t + j * T];
+      a1[j] = r1[t + j * T];
+    }
   }
   constexpr int kBlock = ROWS * T;
   const double* vd0 = P.v_dense[0];
@@ -400,10 +436,18 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
     a1[j] = cscale(cmul(p1, u1), cf);
   }
   fft_run2<N, E, false, V>(a0, a1, t, tw, lds0, lds1);
+  if (stream) {
 #pragma unroll
-  for (int j = 0; j < E; ++j) {
-    r0[t + j * T] = a0[j];
-    r1[t + j * T] = a1[j];
+    for (int j = 0; j < E; ++j) {
+      st_stream(r0 + t + j * T, a0[j]);
+      st_stream(r1 + t + j * T, a1[j]);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      r0[t + j * T] = a0[j];
+      r1[t + j * T] = a1[j];
+    }
   }
 }
 

@@ -12,9 +12,9 @@ namespace sgpe {
 // Split-step pass launchers; with non-null events the launch carries them
 // (hipExtLaunchKernelGGL), which timestamps the kernel's own execution.
 typedef void (*ColFn)(dim3, dim3, size_t, hipStream_t, hipEvent_t, hipEvent_t, const InstDev*, RunDev*, const void*,
-                      int, int, int);
+                      int, int, int, int);
 typedef void (*RowFn)(dim3, dim3, size_t, hipStream_t, hipEvent_t, hipEvent_t, const InstDev*, RunDev*, const void*,
-                      int, int, int);
+                      int, int, int, int);
 typedef void (*FRowFn)(dim3, dim3, size_t, hipStream_t, const void*, void*, int, int, double, const void*);
 typedef void (*FColFn)(dim3, dim3, size_t, hipStream_t, const void*, void*, int, int, const void*);
 typedef void (*EFn)(dim3, dim3, size_t, hipStream_t, const InstDev*, const RunDev*, const void*);
@@ -37,23 +37,23 @@ KTable registry_f32(int kind);
 
 template <typename R, int N, int E, int W>
 void col_launch(dim3 g, dim3 b, size_t l, hipStream_t s, hipEvent_t e0, hipEvent_t e1, const InstDev* i, RunDev* r,
-                const void* tw, int m, int dp, int dc) {
+                const void* tw, int m, int dp, int dc, int st) {
   typedef typename Cx<R>::T C;
   if (e0)
     hipExtLaunchKernelGGL((col_pass_kernel<R, N, E, W>), g, b, (uint32_t)l, s, e0, e1, 0, i, r, (const C*)tw, m, dp,
-                          dc);
+                          dc, st);
   else
-    hipLaunchKernelGGL((col_pass_kernel<R, N, E, W>), g, b, l, s, i, r, (const C*)tw, m, dp, dc);
+    hipLaunchKernelGGL((col_pass_kernel<R, N, E, W>), g, b, l, s, i, r, (const C*)tw, m, dp, dc, st);
 }
 template <typename R, int N, int E, int ROWS>
 void row_launch(dim3 g, dim3 b, size_t l, hipStream_t s, hipEvent_t e0, hipEvent_t e1, const InstDev* i, RunDev* r,
-                const void* tw, int dur, int first, int colw) {
+                const void* tw, int dur, int first, int colw, int st) {
   typedef typename Cx<R>::T C;
   if (e0)
     hipExtLaunchKernelGGL((row_pass_kernel<R, N, E, ROWS>), g, b, (uint32_t)l, s, e0, e1, 0, i, r, (const C*)tw, dur,
-                          first, colw);
+                          first, colw, st);
   else
-    hipLaunchKernelGGL((row_pass_kernel<R, N, E, ROWS>), g, b, l, s, i, r, (const C*)tw, dur, first, colw);
+    hipLaunchKernelGGL((row_pass_kernel<R, N, E, ROWS>), g, b, l, s, i, r, (const C*)tw, dur, first, colw, st);
 }
 template <typename R, int N, int E, int ROWS>
 void frow_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const void* in, void* out, int inv, int shift, double scale,

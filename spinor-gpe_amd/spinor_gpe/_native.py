@@ -78,6 +78,8 @@ def _declare(lib):
         'sgpe_plan_tune': (i, [vp, i, i, i, i]),
         'sgpe_list_shapes': (i, [i, i, i, ip, i]),
         'sgpe_plan_get_tuning': (i, [vp, ip]),
+        'sgpe_plan_set_streaming': (i, [vp, i, i]),
+        'sgpe_plan_get_streaming': (i, [vp, ip]),
         'sgpe_single_step': (i, [vp, i]),
         'sgpe_full_steps': (i, [vp, i]),
         'sgpe_graph_steps': (i, [vp, i, i]),

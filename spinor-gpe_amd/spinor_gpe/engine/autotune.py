@@ -2,13 +2,16 @@
 
 For a grid length the library compiles several shapes of each pass: E
 elements per thread (the FFT's radix) and the number of rows (row pass) or
-columns (column pass) per workgroup. Which one is fastest depends on the
-grid, the precision and how many instances share a launch (small single
-instances are latency-bound and want many small waves; large or batched ones
-want fewer, fatter threads). The tuner times every compiled shape with
-``sgpe_profile_full_steps`` on a scratch plan (the caller's states are never
-touched), picks the fastest row and column shapes independently and caches
-the choice in a JSON file keyed by precision, grid and batch-size bucket.
+columns (column pass) per workgroup; each pass can also stream its state
+accesses past the caches (non-temporal) or not. Which choice is fastest
+depends on the grid, the precision and how many instances share a launch
+(small single instances are latency-bound and want many small waves and
+streaming; large or batched ones want fewer, fatter threads). The tuner times
+every compiled shape in both cache modes with ``sgpe_profile_full_steps`` on
+a scratch plan (the caller's states are never touched), picks the fastest
+row and column choices independently and caches them in a JSON file keyed by
+precision, grid and batch-size bucket as
+``[row E, rows, column E, columns, row streaming, column streaming]``.
 
 Usage: ``python -m spinor_gpe.engine.autotune --grids 128 256 512 1024 2048``
 writes the cache (``$SGPE_TUNE_FILE``, default
@@ -56,7 +59,8 @@ def _load(path):
 
 
 def lookup(precision, ny, nx, n_slots):
-    """Cached (row_e, row_rows, col_e, col_w) or None (user cache, then packaged)."""
+    """Cached (row_e, row_rows, col_e, col_w[, row_stream, col_stream]) or
+    None (user cache, then packaged)."""
     k = key(precision, ny, nx, n_slots)
     for path in (cache_path(), PACKAGED):
         entry = _load(path).get(k)
@@ -118,32 +122,43 @@ def tune(device, precision, ny, nx, n_slots=1, steps=30, reps=3, verbose=False):
     for s in range(n_slots):
         plan.set_slot(s, spec, states[s])
     base = plan.tuning()
+    base_st = plan.streaming()
     timings = {'row': {}, 'col': {}}
-    best = list(base)
+    best = list(base) + list(base_st)
     for pass_, name in ((0, 'row'), (1, 'col')):
         best_t = float('inf')
-        for e, p in shapes(pass_, precision, nx if pass_ == 0 else ny):
-            try:
-                if pass_ == 0:
-                    plan.tune(row_e=e, row_rows=p, col_e=base[2], col_w=base[3])
-                else:
-                    plan.tune(row_e=base[0], row_rows=base[1], col_e=e, col_w=p)
-            except nat.SgpeError:
-                continue  # exceeds LDS or does not tile this grid
-            plan.profile_steps(2)
-            idx = 1 if pass_ == 0 else 0  # profile_steps -> (col ms, row ms, ...)
-            t = min(plan.profile_steps(steps)[idx] for _ in range(reps)) / (3 * steps) * 1e3
-            timings[name][f"{e}x{p}"] = round(t, 3)
-            if verbose:
-                print(f"  {name} E={e} {p}: {t:.2f} us", flush=True)
-            if t < best_t:
-                best_t = t
-                if pass_ == 0:
-                    best[0], best[1] = e, p
-                else:
-                    best[2], best[3] = e, p
-    plan.tune(*best)
+        for st in (0, 1):
+            for e, p in shapes(pass_, precision, nx if pass_ == 0 else ny):
+                try:
+                    if pass_ == 0:
+                        plan.tune(row_e=e, row_rows=p, col_e=base[2], col_w=base[3])
+                        plan.set_streaming(st, base_st[1])
+                    else:
+                        plan.tune(row_e=base[0], row_rows=base[1], col_e=e, col_w=p)
+                        plan.set_streaming(base_st[0], st)
+                except nat.SgpeError:
+                    continue  # exceeds LDS or does not tile this grid
+                plan.profile_steps(2)
+                idx = 1 if pass_ == 0 else 0  # profile_steps -> (col ms, row ms, ...)
+                t = min(plan.profile_steps(steps)[idx] for _ in range(reps)) / (3 * steps) * 1e3
+                timings[name][f"{e}x{p}" + ("s" if st else "")] = round(t, 3)
+                if verbose:
+                    print(f"  {name} E={e} {p}{' streaming' if st else ''}: {t:.2f} us", flush=True)
+                if t < best_t:
+                    best_t = t
+                    if pass_ == 0:
+                        best[0], best[1], best[4] = e, p, st
+                    else:
+                        best[2], best[3], best[5] = e, p, st
+    apply_shape(plan, best)
     return tuple(best), timings
+
+
+def apply_shape(plan, shape):
+    """Set a cached choice: 4 shape entries, optionally 2 streaming flags."""
+    plan.tune(*shape[:4])
+    if len(shape) >= 6:
+        plan.set_streaming(shape[4], shape[5])
 
 
 def apply(plan):
@@ -151,7 +166,7 @@ def apply(plan):
     shape = lookup(plan.prec, plan.ny, plan.nx, plan.n_slots)
     if shape:
         try:
-            plan.tune(*shape)
+            apply_shape(plan, shape)
         except nat.SgpeError:
             return None
     return shape

@@ -190,8 +190,18 @@ class Plan:
         self._call('sgpe_plan_tune', int(row_e), int(row_rows), int(col_e), int(col_w))
 
     def tuning(self):
+        """(row E, rows per block, column E, columns per block)."""
         out = (ctypes.c_int * 4)()
         nat.check(nat.lib().sgpe_plan_get_tuning(self.handle, out))
+        return tuple(out)
+
+    def set_streaming(self, row_stream, col_stream):
+        """Non-temporal (1) or normal (0) state accesses of each pass."""
+        self._call('sgpe_plan_set_streaming', int(bool(row_stream)), int(bool(col_stream)))
+
+    def streaming(self):
+        out = (ctypes.c_int * 2)()
+        nat.check(nat.lib().sgpe_plan_get_streaming(self.handle, out))
         return tuple(out)
 
     def single_step(self, which):

@@ -47,7 +47,8 @@ def main(argv=None):
     plan.profile_steps(2)
     col_ms, row_ms, n_col, n_row = plan.profile_steps(args.steps)
     print(json.dumps({'grid': n, 'precision': args.precision, 'slots': args.slots,
-                      'shape': plan.tuning(), 'col_pass_us': col_ms / n_col * 1e3,
+                      'shape': plan.tuning(), 'streaming': plan.streaming(),
+                      'col_pass_us': col_ms / n_col * 1e3,
                       'row_pass_us': row_ms / n_row * 1e3}), flush=True)
     return 0
 
