@@ -272,3 +272,74 @@ def test_unwrap_recovers_ramp():
     assert abs(d.flat[0] / (2 * np.pi) - round(d.flat[0] / (2 * np.pi))) < 1e-12
     small = 0.3 * np.sin(xx / 5.0) * np.cos(yy / 7.0)
     np.testing.assert_array_equal(oe.unwrap_phase_py(small), small)
+
+
+# ------------------------------------------------------- frames of the Raman coupling
+
+WAVEL = 790.1e-9
+
+
+def gauge_pair(kl_rot_sign, time, seed=21):
+    """A lab-frame instance (theta = 2 kL x, tensor_propagator.py:126-129;
+    unshifted dispersions) and a rotated-frame one (theta = 0, dispersions
+    shifted by +/- kx kL', pspinor.py:493-501) with kL' = kl_rot_sign * kL,
+    the rotated state being the lab state times exp(+i kL x) (component 0)
+    and exp(-i kL x) (component 1). kL is a multiple of the grid's dk, so
+    the phase factors are exact cyclic shifts in k; the grid is fine enough
+    (k_max = 16 kL) that the state carries no weight where the shift wraps."""
+    pss = []
+    for i, (sign, frame) in enumerate(((1.0, 'lab'), (kl_rot_sign, 'rot'))):
+        ps = make(mesh=(256, 256), r=(8, 8), atoms=2e3, name=f'gauge-{frame}-{kl_rot_sign}-{time}-{seed}')
+        dk = ps.space['dk'][0]
+        scale = 8 * dk / (np.sqrt(2) * np.pi / WAVEL * ps.a_x)
+        ps.coupling_setup(wavel=WAVEL, scale=sign * scale, kin_shift=(frame == 'rot'))
+        ps.coupling_uniform(3.0)
+        ps.detuning_grad(0.4, 0.2, axis=0)  # breaks x -> -x symmetry
+        ps.rot_coupling = frame == 'rot'
+        pss.append(ps)
+    lab, rot = pss
+    kl = 8 * lab.space['dk'][0]
+    assert lab.kL_recoil == pytest.approx(kl)
+    # smooth (Gaussian) components: negligible weight near the k-grid edge,
+    # where the cyclic shift by kL would wrap
+    rng = np.random.default_rng(seed)
+    psi = [packet(lab, k0=tuple(rng.uniform(-0.5, 0.5, 2)), width=1.2,
+                  center=tuple(rng.uniform(-0.5, 0.5, 2))) for _ in range(2)]
+    x = lab.space['x_mesh']
+    set_psi(lab, psi)
+    set_psi(rot, [psi[0] * np.exp(1j * kl * x), psi[1] * np.exp(-1j * kl * x)])
+    return lab, rot, kl
+
+
+@pytest.mark.parametrize('time', ['real', 'imag'])
+def test_lab_and_rotated_frames_are_one_gauge(time):
+    """The reference's two Raman frames are related by a gauge transform:
+    the lab frame with recoil +kL equals the rotated frame with recoil -kL
+    (the rotated frame's dispersion shift, T_0 = T + kx kL, pairs with the
+    lab frame's exp(-i theta) on the (0, 1) coupling entry only with the
+    opposite recoil sign). This pins both sign conventions together: a
+    flipped theta or a flipped shift breaks it, and with the recoils of equal
+    sign the frames disagree."""
+    dt = 1e-3 if time == 'real' else 5e-3
+    lab, rot, kl = gauge_pair(-1.0, time)
+    x = lab.space['x_mesh']
+    pl = rp.OraclePropagator(lab, dt, time=time)
+    pr = rp.OraclePropagator(rot, dt, time=time)
+    pl.run(40)
+    pr.run(40)
+    psi_l = oe.ifft_2d_np(pl.psik_numpy(), lab.space['dr'])
+    psi_r = oe.ifft_2d_np(pr.psik_numpy(), rot.space['dr'])
+    want = [psi_l[0] * np.exp(1j * kl * x), psi_l[1] * np.exp(-1j * kl * x)]
+    assert rel(psi_r, want) < 1e-9
+    # the same recoil sign in both frames is a different system
+    if time == 'imag':
+        return
+    lab2, rot2, _ = gauge_pair(+1.0, time)
+    pl2 = rp.OraclePropagator(lab2, dt, time=time)
+    pr2 = rp.OraclePropagator(rot2, dt, time=time)
+    pl2.run(10)
+    pr2.run(10)
+    psi_l2 = oe.ifft_2d_np(pl2.psik_numpy(), lab2.space['dr'])
+    psi_r2 = oe.ifft_2d_np(pr2.psik_numpy(), rot2.space['dr'])
+    want2 = [psi_l2[0] * np.exp(1j * kl * x), psi_l2[1] * np.exp(-1j * kl * x)]
+    assert rel(psi_r2, want2) > 1e-3
