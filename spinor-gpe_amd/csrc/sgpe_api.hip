@@ -180,6 +180,13 @@ static size_t lds_bytes(size_t data_bytes, int n, size_t cs, int tables = 0) {
 }
 static const int kColTables = 2;
 
+// Dynamic LDS of a row pass: exchange buffers and twiddles, plus a second
+// exchange buffer when the kernel double-buffers (row_double_buffer).
+static size_t row_lds_bytes(int e, size_t data_bytes, int n, size_t cs) {
+  const size_t tw = tw_in_lds(data_bytes, n, cs) ? (size_t)n * cs : 0;
+  return lds_bytes(data_bytes, n, cs) + (row_double_buffer(e, data_bytes, tw) ? data_bytes : 0);
+}
+
 static int ctx_reduction(sgpe_ctx* ctx, size_t count) {
   if (ctx->red_count >= count) return SGPE_OK;
   if (ctx->red) HIP_TRY(hipFree(ctx->red));
@@ -237,7 +244,7 @@ static int plan_select(sgpe_plan* p, int row_e, int row_rows, int col_e, int col
   const size_t row_data = (size_t)row_rows * 2 * p->nx * cs;
   const size_t col_data = (size_t)col_w * p->ny * cs;
   if (row_data > kMaxLds || col_data > kMaxLds) return fail(SGPE_E_UNSUPPORTED, "FFT shape exceeds LDS");
-  int rc = set_lds_attr(r->kernel, lds_bytes(row_data, p->nx, cs));
+  int rc = set_lds_attr(r->kernel, row_lds_bytes(row_e, row_data, p->nx, cs));
   if (rc) return rc;
   rc = set_lds_attr(c->kernel, lds_bytes(col_data, p->ny, cs, kColTables));
   if (rc) return rc;
@@ -266,7 +273,7 @@ static void launch_col(sgpe_plan* p, hipStream_t s, int mode, int dprev, int dcu
 
 static void launch_row(sgpe_plan* p, hipStream_t s, int dur, int first, hipEvent_t e0 = nullptr,
                        hipEvent_t e1 = nullptr) {
-  const size_t lds = lds_bytes((size_t)p->row_rows * 2 * p->nx * csize(p->prec), p->nx, csize(p->prec));
+  const size_t lds = row_lds_bytes(p->row_e, (size_t)p->row_rows * 2 * p->nx * csize(p->prec), p->nx, csize(p->prec));
   dim3 grid(p->ny / p->row_rows, 1, p->nslots), block(p->row_rows * (p->nx / p->row_e));
   ((RowFn)p->row->launch)(grid, block, lds, s, e0, e1, p->d_inst, p->d_run, p->tw_x, dur, first, p->col_w,
                           p->row_stream);

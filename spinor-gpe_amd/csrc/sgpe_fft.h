@@ -205,6 +205,47 @@ __device__ __forceinline__ void fft_run2(C (&a)[E], C (&b)[E], int t, const C* t
   }
 }
 
+
+// Double-buffered variant of fft_run2: stage s writes the buffer pair
+// (l0, l1) or (m0, m1) alternately, so a buffer is rewritten only after the
+// next exchange's barrier, which every thread reaches after its reads of
+// that buffer: one barrier per exchange instead of two.
+template <int N, int E, bool INV, class V, int L = 1, typename C>
+__device__ __forceinline__ void fft_run2_db(C (&a)[E], C (&b)[E], int t, const C* tw, C* l0, C* l1, C* m0, C* m1) {
+  constexpr int R = (N / L >= E) ? E : N / L;
+  constexpr bool LAST = (L * R == N);
+  constexpr int T = N / E;
+  fft_stage<N, E, R, L, INV, LAST, V>(a, t, tw, l0);
+  fft_stage<N, E, R, L, INV, LAST, V>(b, t, tw, l1);
+  if constexpr (!LAST) {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      a[j] = l0[V::at(t + j * T)];
+      b[j] = l1[V::at(t + j * T)];
+    }
+    fft_run2_db<N, E, INV, V, L * R>(a, b, t, tw, m0, m1, l0, l1);
+  }
+}
+
+// Number of LDS exchanges of a transform (stages - 1).
+template <int N, int E, int L = 1> constexpr int fft_exchanges() {
+  constexpr int R = (N / L >= E) ? E : N / L;
+  if constexpr (L * R == N)
+    return 0;
+  else
+    return 1 + fft_exchanges<N, E, L * R>();
+}
+
+// Whether a row pass double-buffers its exchanges: for small radices (many
+// stages, where the saved barriers matter) and only while the second buffer
+// keeps the kernel within 64 KiB of LDS (more would cost occupancy).
+// Same predicate on host and device.
+constexpr size_t kDbLdsBytes = 64 * 1024;
+constexpr bool row_double_buffer(int e, size_t data_bytes, size_t other_bytes) {
+  return e <= 4 && 2 * data_bytes + other_bytes <= kDbLdsBytes;
+}
+
 // Twiddle table for the FFT stages: copies the n - 1 stage-major entries to
 // LDS at `dst` when IN_LDS (the first exchange barrier of fft_run orders the
 // copy before any read) and returns the table to read from.

@@ -266,10 +266,12 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
   typedef typename Cx<R>::T C;
   typedef LdsView<1, 1 << 30, 0> V;
   constexpr int T = N / E;
-  // LDS: [ROWS*2*N exchange][N twiddles]. The x phase tables stay in
-  // global memory (cache resident): copies in LDS would cost occupancy.
+  // LDS: [ROWS*2*N exchange][N twiddles][ROWS*2*N second exchange buffer,
+  // when double-buffered]. The x phase tables stay in global memory (cache
+  // resident): copies in LDS would cost occupancy.
   constexpr size_t kData = (size_t)ROWS * 2 * N * sizeof(C);
   constexpr bool kTw = tw_in_lds(kData, N, sizeof(C));
+  constexpr bool kDb = row_double_buffer(E, kData, kTw ? (size_t)N * sizeof(C) : 0);
   extern __shared__ double smem[];
   const int inst = blockIdx.z;
   RunDev& run = runs[inst];
@@ -352,7 +354,12 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
   if (blockIdx.x == 0 && threadIdx.x == 0) run.form = FORM_HYBRID;
   const double lam2 = P.atom_num / (P.dv_r * (double)N * (double)ny * nrm[0]);
 
-  fft_run2<N, E, true, V>(a0, a1, t, tw, lds0, lds1);
+  C* const alt0 = sm + ROWS * 2 * N + (kTw ? N : 0) + (size_t)row * 2 * N;
+  C* const alt1 = alt0 + N;
+  if constexpr (kDb)
+    fft_run2_db<N, E, true, V>(a0, a1, t, tw, lds0, lds1, alt0, alt1);
+  else
+    fft_run2<N, E, true, V>(a0, a1, t, tw, lds0, lds1);
 
   const bool imag = P.imag != 0;
   const R lam = (R)sqrt(lam2);
@@ -435,7 +442,14 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
     a0[j] = cscale(cmul(p0, u0), cf);
     a1[j] = cscale(cmul(p1, u1), cf);
   }
-  fft_run2<N, E, false, V>(a0, a1, t, tw, lds0, lds1);
+  // The forward transform starts on the buffer pair the inverse one read
+  // last but one, so no barrier is needed between them.
+  if constexpr (kDb && fft_exchanges<N, E>() % 2)
+    fft_run2_db<N, E, false, V>(a0, a1, t, tw, alt0, alt1, lds0, lds1);
+  else if constexpr (kDb)
+    fft_run2_db<N, E, false, V>(a0, a1, t, tw, lds0, lds1, alt0, alt1);
+  else
+    fft_run2<N, E, false, V>(a0, a1, t, tw, lds0, lds1);
   if (stream) {
 #pragma unroll
     for (int j = 0; j < E; ++j) {
