@@ -209,6 +209,9 @@ struct sgpe_plan {
   int prec = 0, ny = 0, nx = 0, nslots = 0;
   int row_e = 8, row_rows = 1, col_e = 8, col_w = 4;
   int row_stream = 0, col_stream = 0;  // non-temporal state accesses per pass
+  unsigned epoch = 0;                  // bumped whenever captured graphs go stale
+  bool dense_kin = false;              // some slot has a dense kinetic array
+  bool general_row = false;            // some slot has a dense potential or coupling array
   const KEntry* row = nullptr;
   const KEntry* col = nullptr;
   const void* tw_x = nullptr;  // twiddles for nx (rows)
@@ -233,6 +236,7 @@ static void drop_graphs(sgpe_plan* p) {
   for (auto& g : p->graph_defs) (void)hipGraphDestroy(g.second);
   p->graphs.clear();
   p->graph_defs.clear();
+  ++p->epoch;
 }
 
 static int plan_select(sgpe_plan* p, int row_e, int row_rows, int col_e, int col_w) {
@@ -244,9 +248,11 @@ static int plan_select(sgpe_plan* p, int row_e, int row_rows, int col_e, int col
   const size_t row_data = (size_t)row_rows * 2 * p->nx * cs;
   const size_t col_data = (size_t)col_w * p->ny * cs;
   if (row_data > kMaxLds || col_data > kMaxLds) return fail(SGPE_E_UNSUPPORTED, "FFT shape exceeds LDS");
-  int rc = set_lds_attr(r->kernel, row_lds_bytes(row_e, row_data, p->nx, cs));
-  if (rc) return rc;
-  rc = set_lds_attr(c->kernel, lds_bytes(col_data, p->ny, cs, kColTables));
+  const size_t rl = row_lds_bytes(row_e, row_data, p->nx, cs), cl = lds_bytes(col_data, p->ny, cs, kColTables);
+  int rc = set_lds_attr(r->kernel, rl);
+  if (!rc) rc = set_lds_attr(r->kernel_g, rl);
+  if (!rc) rc = set_lds_attr(c->kernel, cl);
+  if (!rc) rc = set_lds_attr(c->kernel_g, cl);
   if (rc) return rc;
   const void *twx = nullptr, *twy = nullptr;
   if ((rc = ctx_twiddles(p->ctx, p->prec, p->nx, row_e, &twx))) return rc;
@@ -267,16 +273,16 @@ static void launch_col(sgpe_plan* p, hipStream_t s, int mode, int dprev, int dcu
                        hipEvent_t e1 = nullptr) {
   const size_t lds = lds_bytes((size_t)p->col_w * p->ny * csize(p->prec), p->ny, csize(p->prec), kColTables);
   dim3 grid(p->nx / p->col_w, 2, p->nslots), block(p->col_w * (p->ny / p->col_e));
-  ((ColFn)p->col->launch)(grid, block, lds, s, e0, e1, p->d_inst, p->d_run, p->tw_y, mode, dprev, dcur,
-                          p->col_stream);
+  ((ColFn)(p->dense_kin ? p->col->launch_g : p->col->launch))(grid, block, lds, s, e0, e1, p->d_inst, p->d_run,
+                                                              p->tw_y, mode, dprev, dcur, p->col_stream);
 }
 
 static void launch_row(sgpe_plan* p, hipStream_t s, int dur, int first, hipEvent_t e0 = nullptr,
                        hipEvent_t e1 = nullptr) {
   const size_t lds = row_lds_bytes(p->row_e, (size_t)p->row_rows * 2 * p->nx * csize(p->prec), p->nx, csize(p->prec));
   dim3 grid(p->ny / p->row_rows, 1, p->nslots), block(p->row_rows * (p->nx / p->row_e));
-  ((RowFn)p->row->launch)(grid, block, lds, s, e0, e1, p->d_inst, p->d_run, p->tw_x, dur, first, p->col_w,
-                          p->row_stream);
+  ((RowFn)(p->general_row ? p->row->launch_g : p->row->launch))(grid, block, lds, s, e0, e1, p->d_inst, p->d_run,
+                                                                p->tw_x, dur, first, p->col_w, p->row_stream);
 }
 
 static void issue_sub_step(sgpe_plan* p, hipStream_t s, int which, int first) {
@@ -313,6 +319,7 @@ struct sgpe_group {
   hipEvent_t fork = nullptr;
   std::map<long, hipGraphExec_t> graphs;
   std::map<long, hipGraph_t> defs;
+  std::vector<unsigned> epochs;  // plan epochs the cached graphs were captured at
 };
 
 // Energy work buffer and kernel shapes of a plan (allocates once; must not
@@ -460,8 +467,10 @@ static int plan_load_slot(sgpe_plan* p, int slot, const sgpe_inst_desc* d, void*
   if (d->theta_x) {
     o_thx = st.add(nx * sizeof(double));
     std::memcpy(st.bytes.data() + o_thx, d->theta_x, nx * sizeof(double));
-    o_eth = st.add(nx * cs);
   }
+  // e^{-i theta(x)} for the coupling's off-diagonal; all ones without theta
+  // (rotated frame), so the kernels apply it unconditionally.
+  o_eth = st.add(nx * cs);
   for (int dd = 0; dd < 2; ++dd)
     for (int s = 0; s < 2; ++s) {
       o_kx[dd][s] = o_ky[dd][s] = o_px[dd][s] = o_py[dd][s] = 0;
@@ -506,7 +515,7 @@ static int plan_load_slot(sgpe_plan* p, int slot, const sgpe_inst_desc* d, void*
     }
     // Uniform coupling: angle a = Omega Delta/4 for both durations
     // (tensor_propagator.py:142-149 with tensor_tools.py:586).
-    const double a = d->omega * dt * 0.25;
+    const double a = (d->coupling == SGPE_COUPLING_UNIFORM ? d->omega : 0.0) * dt * 0.25;
     if (imag) {
       h.c_alpha[dd] = std::cosh(a);
       h.c_beta_re[dd] = -std::sinh(a);
@@ -517,8 +526,12 @@ static int plan_load_slot(sgpe_plan* p, int slot, const sgpe_inst_desc* d, void*
       h.c_beta_im[dd] = -std::sin(a);
     }
   }
-  if (d->theta_x)
-    for (int i = 0; i < nx; ++i) put_cplx(st, o_eth, prec, i, std::cos(d->theta_x[i]), -std::sin(d->theta_x[i]));
+  for (int i = 0; i < nx; ++i) {
+    if (d->theta_x)
+      put_cplx(st, o_eth, prec, i, std::cos(d->theta_x[i]), -std::sin(d->theta_x[i]));
+    else
+      put_cplx(st, o_eth, prec, i, 1.0, 0.0);
+  }
 
   Slot& sl = p->slots[slot];
   if (sl.bytes < st.bytes.size()) {
@@ -560,6 +573,19 @@ static int plan_load_slot(sgpe_plan* p, int slot, const sgpe_inst_desc* d, void*
   HIP_TRY(hipStreamSynchronize(s));  // staging buffer goes out of scope
   p->h_inst[slot] = h;
   sl.loaded = true;
+  // Kernel variants: the dense-operator ones once any loaded slot needs them.
+  bool dk = false, gr = false;
+  for (int i = 0; i < p->nslots; ++i) {
+    if (!p->slots[i].loaded) continue;
+    const InstDev& q = p->h_inst[i];
+    dk = dk || q.t_dense[0] || q.t_dense[1];
+    gr = gr || q.v_dense[0] || q.v_dense[1] || q.coupling == SGPE_COUPLING_ARRAY;
+  }
+  if (dk != p->dense_kin || gr != p->general_row) {
+    p->dense_kin = dk;
+    p->general_row = gr;
+    drop_graphs(p);
+  }
   return SGPE_OK;
 }
 
@@ -1049,6 +1075,17 @@ int sgpe_group_run(sgpe_group* g, int k, int check, int n_graphs) {
     if (p->last_dur != DUR_OUT) issue_sync(p, g->ctx->stream);
   }
   if (n_graphs == 0) return SGPE_OK;
+  // A plan re-tuned, re-streamed or switched kernel variants since the
+  // capture: its kernels in the cached graphs are stale.
+  std::vector<unsigned> now;
+  for (auto* p : g->plans) now.push_back(p->epoch);
+  if (now != g->epochs) {
+    for (auto& x : g->graphs) (void)hipGraphExecDestroy(x.second);
+    for (auto& x : g->defs) (void)hipGraphDestroy(x.second);
+    g->graphs.clear();
+    g->defs.clear();
+    g->epochs = now;
+  }
   const long key = 2L * k + (check ? 1 : 0);
   auto it = g->graphs.find(key);
   if (it == g->graphs.end()) {

@@ -102,14 +102,22 @@ template <> __device__ __forceinline__ double ex<double>(double x) { return exp(
 template <> __device__ __forceinline__ float ex<float>(float x) { return expf(x); }
 
 // exp(-i E Delta) for Delta = dt (real time) or -i dt (imaginary time).
-template <typename R>
-__device__ __forceinline__ typename Cx<R>::T phase_op(R e_dt, bool imag) {
+template <typename R, bool IMAG>
+__device__ __forceinline__ typename Cx<R>::T phase_op(R e_dt) {
   typedef typename Cx<R>::T C;
-  if (imag) return C{ex<R>(-e_dt), (R)0};
-  R s, c;
-  sc<R>(-e_dt, &s, &c);
-  return C{c, s};
+  if constexpr (IMAG) {
+    return C{ex<R>(-e_dt), (R)0};
+  } else {
+    R s, c;
+    sc<R>(-e_dt, &s, &c);
+    return C{c, s};
+  }
 }
+
+// The pass kernels select their pointwise code by uniform flags (time mode,
+// dense or separable operators) OUTSIDE the loops, in separately compiled
+// variants: inside one loop the compiler would evaluate the sin/cos or exp
+// of every alternative for every point and select afterwards.
 
 // State accesses of the split-step passes with a non-temporal hint: every
 // pass reads and writes each state element exactly once, and streaming them
@@ -146,7 +154,44 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
   return n % kXcds == 0 ? (b % kXcds) * (n / kXcds) + b / kXcds : b;
 }
 
-template <typename R, int N, int E, int W>
+// Kinetic factors of the column pass on this thread's E points of column jx:
+// the closing K of the previous sub-step (unless the state is already in
+// k-space) with the S sum after it, then the opening K of this sub-step with
+// the NRM sum after it (mid). DENSE: K from the dense T array, else from the
+// separable tables.
+template <typename R, int E, int T, bool DENSE, bool IMAG, typename C>
+__device__ __forceinline__ void col_kinetic(C (&a)[E], int t, int nx, int jx, const double* __restrict__ td, R hp,
+                                            R hc, C kxp, C kxc, const C* kyp, const C* kyc, bool kspace, bool mid,
+                                            double& s_acc, double& n_acc) {
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const int jy = t + j * T;
+    R tval = (R)0;
+    if constexpr (DENSE) tval = (R)td[(size_t)jy * nx + jx];
+    if (!kspace) {
+      C k;
+      if constexpr (DENSE)
+        k = phase_op<R, IMAG>(tval * hp);
+      else
+        k = cmul(kxp, kyp[jy]);
+      a[j] = cmul(a[j], k);
+    }
+    s_acc += norm2d<R>(a[j]);
+    if (mid) {
+      C k;
+      if constexpr (DENSE)
+        k = phase_op<R, IMAG>(tval * hc);
+      else
+        k = cmul(kxc, kyc[jy]);
+      a[j] = cmul(a[j], k);
+      n_acc += norm2d<R>(a[j]);
+    }
+  }
+}
+
+// DENSE: some slot has a dense kinetic array (the kernel then also handles
+// table slots); otherwise every slot uses the separable tables.
+template <typename R, int N, int E, int W, bool DENSE>
 __global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __restrict__ insts,
                                                              RunDev* __restrict__ runs,
                                                              const typename Cx<R>::T* __restrict__ tws,
@@ -207,24 +252,17 @@ __global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __re
   else
     __syncthreads();  // publishes the staged tables
 
-  const bool imag = P.imag != 0;
   double s_acc = 0.0, n_acc = 0.0;
   const R hp = (R)(0.5 * P.dt_sub[d_prev]);
   const R hc = (R)(0.5 * P.dt_sub[d_cur]);
-#pragma unroll
-  for (int j = 0; j < E; ++j) {
-    const int jy = t + j * T;
-    R tval = td ? (R)td[(size_t)jy * nx + jx] : (R)0;
-    if (!kspace) {
-      C k = td ? phase_op<R>(tval * hp, imag) : cmul(kxp, kyp[jy]);
-      a[j] = cmul(a[j], k);
-    }
-    s_acc += norm2d<R>(a[j]);
-    if (mode == COL_MID) {
-      C k = td ? phase_op<R>(tval * hc, imag) : cmul(kxc, kyc[jy]);
-      a[j] = cmul(a[j], k);
-      n_acc += norm2d<R>(a[j]);
-    }
+  const bool mid = mode == COL_MID;
+  if (!DENSE || !td)
+    col_kinetic<R, E, T, false, false>(a, t, nx, jx, nullptr, hp, hc, kxp, kxc, kyp, kyc, kspace, mid, s_acc, n_acc);
+  else if constexpr (DENSE) {
+    if (P.imag)
+      col_kinetic<R, E, T, true, true>(a, t, nx, jx, td, hp, hc, kxp, kxc, kyp, kyc, kspace, mid, s_acc, n_acc);
+    else
+      col_kinetic<R, E, T, true, false>(a, t, nx, jx, td, hp, hc, kxp, kxc, kyp, kyc, kspace, mid, s_acc, n_acc);
   }
   if (mode == COL_MID) fft_run<N, E, true, V>(a, t, tw, lds);
   if (stream) {
@@ -256,9 +294,104 @@ __device__ __forceinline__ void part_sums(const double* part, int nt2, int slot,
   out2[1] = v[1];
 }
 
+// Real-space block of the row pass on this thread's E points of row y, both
+// components: U C P C U with the normalization folded in (tensor_propagator.
+// py:244-267). FAST: separable potential (x and y tables) and no or uniform
+// coupling, which is then always applied (the identity when off) with its
+// e^{-i theta} table (ones in the rotated frame); otherwise dense potentials
+// and coupling arrays where present.
+template <typename R, int N, int E, int T, bool IMAG, bool FAST, typename C>
+__device__ __forceinline__ void row_pointwise(C (&a0)[E], C (&a1)[E], const InstDev& P, int dur, int y, int t,
+                                              double lam2, C py0, C py1, const C (&tx0)[E], const C (&tx1)[E],
+                                              const C (&tex)[E]) {
+  const R lam = (R)sqrt(lam2);
+  const R lam2r = (R)lam2;
+  const R cf = (R)P.fft_c;
+  const R guu = (R)P.g_uu, gdd = (R)P.g_dd, gud = (R)P.g_ud;
+  const R hdt = (R)(0.5 * P.dt_sub[dur]);
+  const R dts = (R)P.dt_sub[dur];
+  const int cpl = P.coupling;
+  const R ca = (R)P.c_alpha[dur];
+  const C cb = C{(R)P.c_beta_re[dur], (R)P.c_beta_im[dur]};
+  const double* omd = P.omega_dense;
+  const double* vd0 = P.v_dense[0];
+  const double* vd1 = P.v_dense[1];
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const int x = t + j * T;
+    C p0 = a0[j], p1 = a1[j];
+    // Interaction half step U_s = exp(-i (g n) Delta/2) from the normalized
+    // densities (tensor_propagator.py:247-250); U is reused after C P C (:267).
+    const R n0 = lam2r * (p0.x * p0.x + p0.y * p0.y);
+    const R n1 = lam2r * (p1.x * p1.x + p1.y * p1.y);
+    const C u0 = phase_op<R, IMAG>((guu * n0 + gud * n1) * hdt);
+    const C u1 = phase_op<R, IMAG>((gdd * n1 + gud * n0) * hdt);
+    p0 = cscale(cmul(p0, u0), lam);
+    p1 = cscale(cmul(p1, u1), lam);
+    R alpha = ca;
+    C b01 = cb, b10 = cb;
+    bool apply = true;
+    if constexpr (FAST) {
+      const C e = tex[j];
+      b10 = cmul(b01, cconj(e));
+      b01 = cmul(b01, e);
+    } else {
+      apply = cpl != 0;
+      if (cpl == 2) {
+        const R ang = (R)omd[(size_t)y * N + x] * dts * (R)0.25;
+        if constexpr (IMAG) {
+          const R e = ex<R>(ang), ei = (R)1 / e;
+          alpha = (R)0.5 * (e + ei);
+          b01 = C{(R)-0.5 * (e - ei), (R)0};
+        } else {
+          R s, c;
+          sc<R>(ang, &s, &c);
+          alpha = c;
+          b01 = C{(R)0, -s};
+        }
+        b10 = b01;
+      }
+      if (cpl) {
+        const C e = tex[j];
+        b10 = cmul(b01, cconj(e));
+        b01 = cmul(b01, e);
+      }
+    }
+    // Coupling half step C (tensor_propagator.py:252-254).
+    if (apply) {
+      const C q0 = cadd(cscale(p0, alpha), cmul(b01, p1));
+      const C q1 = cadd(cmul(b10, p0), cscale(p1, alpha));
+      p0 = q0;
+      p1 = q1;
+    }
+    // Potential step P_s = exp(-i V_s Delta) (tensor_propagator.py:256).
+    C q0, q1;
+    if constexpr (FAST) {
+      q0 = cmul(tx0[j], py0);
+      q1 = cmul(tx1[j], py1);
+    } else {
+      q0 = vd0 ? phase_op<R, IMAG>((R)vd0[(size_t)y * N + x] * dts) : cmul(tx0[j], py0);
+      q1 = vd1 ? phase_op<R, IMAG>((R)vd1[(size_t)y * N + x] * dts) : cmul(tx1[j], py1);
+    }
+    p0 = cmul(p0, q0);
+    p1 = cmul(p1, q1);
+    // Second coupling half step (tensor_propagator.py:258-260).
+    if (apply) {
+      const C s0 = cadd(cscale(p0, alpha), cmul(b01, p1));
+      const C s1 = cadd(cmul(b10, p0), cscale(p1, alpha));
+      p0 = s0;
+      p1 = s1;
+    }
+    a0[j] = cscale(cmul(p0, u0), cf);
+    a1[j] = cscale(cmul(p1, u1), cf);
+  }
+}
+
 // ---------------------------------------------------------------- row pass
 
-template <typename R, int N, int E, int ROWS>
+// GENERAL: some slot has a dense potential or a coupling array; otherwise
+// every slot takes the separable fast path.
+template <typename R, int N, int E, int ROWS, bool GENERAL>
 __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* __restrict__ insts,
                                                                 RunDev* __restrict__ runs,
                                                                 const typename Cx<R>::T* __restrict__ tws,
@@ -361,87 +494,24 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
   else
     fft_run2<N, E, true, V>(a0, a1, t, tw, lds0, lds1);
 
-  const bool imag = P.imag != 0;
-  const R lam = (R)sqrt(lam2);
-  const R lam2r = (R)lam2;
-  const R cf = (R)P.fft_c;
-  const R guu = (R)P.g_uu, gdd = (R)P.g_dd, gud = (R)P.g_ud;
-  const R hdt = (R)(0.5 * P.dt_sub[dur]);
-  const R dts = (R)P.dt_sub[dur];
-  const int cpl = P.coupling;
-  const R ca = (R)P.c_alpha[dur];
-  const C cb = C{(R)P.c_beta_re[dur], (R)P.c_beta_im[dur]};
-  const double* omd = P.omega_dense;
-
   // The x-table values of this thread's E points are loaded together, so
-  // the phase waits for memory once rather than once per point. An absent
-  // table is stood in for by the twiddles (N entries; values unused).
+  // the block waits for memory once rather than once per point. An absent
+  // potential table is stood in for by the twiddles (N entries; unused).
   C tx0[E], tx1[E], tex[E];
   {
     const C* s0 = px0 ? px0 : tw;
     const C* s1 = px1 ? px1 : tw;
-    const C* se = eth ? eth : tw;
 #pragma unroll
     for (int j = 0; j < E; ++j) {
       tx0[j] = s0[t + j * T];
       tx1[j] = s1[t + j * T];
-      tex[j] = se[t + j * T];
+      tex[j] = eth[t + j * T];
     }
   }
-
-#pragma unroll
-  for (int j = 0; j < E; ++j) {
-    const int x = t + j * T;
-    C p0 = a0[j], p1 = a1[j];
-    // Interaction half step U_s = exp(-i (g n) Delta/2) from the normalized
-    // densities (tensor_propagator.py:247-250); U is reused after C P C (:267).
-    const R n0 = lam2r * (p0.x * p0.x + p0.y * p0.y);
-    const R n1 = lam2r * (p1.x * p1.x + p1.y * p1.y);
-    const C u0 = phase_op<R>((guu * n0 + gud * n1) * hdt, imag);
-    const C u1 = phase_op<R>((gdd * n1 + gud * n0) * hdt, imag);
-    p0 = cscale(cmul(p0, u0), lam);
-    p1 = cscale(cmul(p1, u1), lam);
-    R alpha = ca;
-    C b01 = cb, b10 = cb;
-    if (cpl) {
-      if (cpl == 2) {
-        const R ang = (R)omd[(size_t)y * N + x] * dts * (R)0.25;
-        if (imag) {
-          const R e = ex<R>(ang), ei = (R)1 / e;
-          alpha = (R)0.5 * (e + ei);
-          b01 = C{(R)-0.5 * (e - ei), (R)0};
-        } else {
-          R s, c;
-          sc<R>(ang, &s, &c);
-          alpha = c;
-          b01 = C{(R)0, -s};
-        }
-        b10 = b01;
-      }
-      if (eth) {
-        const C e = tex[j];
-        b10 = cmul(b01, cconj(e));
-        b01 = cmul(b01, e);
-      }
-      const C q0 = cadd(cscale(p0, alpha), cmul(b01, p1));
-      const C q1 = cadd(cmul(b10, p0), cscale(p1, alpha));
-      p0 = q0;
-      p1 = q1;
-    }
-    // Potential step P_s = exp(-i V_s Delta) (tensor_propagator.py:256).
-    const C q0 = vd0 ? phase_op<R>((R)vd0[(size_t)y * N + x] * dts, imag) : cmul(tx0[j], py0);
-    const C q1 = vd1 ? phase_op<R>((R)vd1[(size_t)y * N + x] * dts, imag) : cmul(tx1[j], py1);
-    p0 = cmul(p0, q0);
-    p1 = cmul(p1, q1);
-    if (cpl) {
-      const C s0 = cadd(cscale(p0, alpha), cmul(b01, p1));
-      const C s1 = cadd(cmul(b10, p0), cscale(p1, alpha));
-      p0 = s0;
-      p1 = s1;
-    }
-    a0[j] = cscale(cmul(p0, u0), cf);
-    a1[j] = cscale(cmul(p1, u1), cf);
-  }
+  if (P.imag)
+    row_pointwise<R, N, E, T, true, !GENERAL>(a0, a1, P, dur, y, t, lam2, py0, py1, tx0, tx1, tex);
+  else
+    row_pointwise<R, N, E, T, false, !GENERAL>(a0, a1, P, dur, y, t, lam2, py0, py1, tx0, tx1, tex);
   // The forward transform starts on the buffer pair the inverse one read
   // last but one, so no barrier is needed between them.
   if constexpr (kDb && fft_exchanges<N, E>() % 2)

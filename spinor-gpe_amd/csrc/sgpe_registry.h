@@ -23,6 +23,8 @@ struct KEntry {
   int prec, n, e, p;  // p = W (columns per block) or ROWS (rows per block)
   void* launch;
   const void* kernel;
+  void* launch_g;  // split-step passes: the variant for dense operators
+  const void* kernel_g;
 };
 
 struct KTable {
@@ -35,25 +37,25 @@ enum { K_COL = 0, K_ROW = 1, K_FROW = 2, K_FCOL = 3, K_ECOL = 4, K_EROW = 5, K_K
 KTable registry_f64(int kind);
 KTable registry_f32(int kind);
 
-template <typename R, int N, int E, int W>
+template <typename R, int N, int E, int W, bool G>
 void col_launch(dim3 g, dim3 b, size_t l, hipStream_t s, hipEvent_t e0, hipEvent_t e1, const InstDev* i, RunDev* r,
                 const void* tw, int m, int dp, int dc, int st) {
   typedef typename Cx<R>::T C;
   if (e0)
-    hipExtLaunchKernelGGL((col_pass_kernel<R, N, E, W>), g, b, (uint32_t)l, s, e0, e1, 0, i, r, (const C*)tw, m, dp,
-                          dc, st);
+    hipExtLaunchKernelGGL((col_pass_kernel<R, N, E, W, G>), g, b, (uint32_t)l, s, e0, e1, 0, i, r, (const C*)tw, m,
+                          dp, dc, st);
   else
-    hipLaunchKernelGGL((col_pass_kernel<R, N, E, W>), g, b, l, s, i, r, (const C*)tw, m, dp, dc, st);
+    hipLaunchKernelGGL((col_pass_kernel<R, N, E, W, G>), g, b, l, s, i, r, (const C*)tw, m, dp, dc, st);
 }
-template <typename R, int N, int E, int ROWS>
+template <typename R, int N, int E, int ROWS, bool G>
 void row_launch(dim3 g, dim3 b, size_t l, hipStream_t s, hipEvent_t e0, hipEvent_t e1, const InstDev* i, RunDev* r,
                 const void* tw, int dur, int first, int colw, int st) {
   typedef typename Cx<R>::T C;
   if (e0)
-    hipExtLaunchKernelGGL((row_pass_kernel<R, N, E, ROWS>), g, b, (uint32_t)l, s, e0, e1, 0, i, r, (const C*)tw, dur,
-                          first, colw, st);
+    hipExtLaunchKernelGGL((row_pass_kernel<R, N, E, ROWS, G>), g, b, (uint32_t)l, s, e0, e1, 0, i, r, (const C*)tw,
+                          dur, first, colw, st);
   else
-    hipLaunchKernelGGL((row_pass_kernel<R, N, E, ROWS>), g, b, l, s, i, r, (const C*)tw, dur, first, colw, st);
+    hipLaunchKernelGGL((row_pass_kernel<R, N, E, ROWS, G>), g, b, l, s, i, r, (const C*)tw, dur, first, colw, st);
 }
 template <typename R, int N, int E, int ROWS>
 void frow_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const void* in, void* out, int inv, int shift, double scale,
@@ -78,18 +80,36 @@ void erow_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const InstDev* i, cons
 
 }  // namespace sgpe
 
-#define SGPE_COL_E(R, PR, N, E, W) \
-  {PR, N, E, W, (void*)&sgpe::col_launch<R, N, E, W>, (const void*)&sgpe::col_pass_kernel<R, N, E, W>},
-#define SGPE_ROW_E(R, PR, N, E, RW) \
-  {PR, N, E, RW, (void*)&sgpe::row_launch<R, N, E, RW>, (const void*)&sgpe::row_pass_kernel<R, N, E, RW>},
-#define SGPE_FROW_E(R, PR, N, E, RW) \
-  {PR, N, E, RW, (void*)&sgpe::frow_launch<R, N, E, RW>, (const void*)&sgpe::fft_rows_kernel<R, N, E, RW>},
-#define SGPE_FCOL_E(R, PR, N, E, W) \
-  {PR, N, E, W, (void*)&sgpe::fcol_launch<R, N, E, W>, (const void*)&sgpe::fft_cols_kernel<R, N, E, W>},
-#define SGPE_ECOL_E(R, PR, N, E, W) \
-  {PR, N, E, W, (void*)&sgpe::ecol_launch<R, N, E, W>, (const void*)&sgpe::energy_col_kernel<R, N, E, W>},
-#define SGPE_EROW_E(R, PR, N, E, RW) \
-  {PR, N, E, RW, (void*)&sgpe::erow_launch<R, N, E, RW>, (const void*)&sgpe::energy_row_kernel<R, N, E, RW>},
+#define SGPE_COL_E(R, PR, N, E, W)                                                                   \
+  {PR,                                                                                               \
+   N,                                                                                                \
+   E,                                                                                                \
+   W,                                                                                                \
+   (void*)&sgpe::col_launch<R, N, E, W, false>,                                                      \
+   (const void*)&sgpe::col_pass_kernel<R, N, E, W, false>,                                           \
+   (void*)&sgpe::col_launch<R, N, E, W, true>,                                                       \
+   (const void*)&sgpe::col_pass_kernel<R, N, E, W, true>},
+#define SGPE_ROW_E(R, PR, N, E, RW)                                                                  \
+  {PR,                                                                                               \
+   N,                                                                                                \
+   E,                                                                                                \
+   RW,                                                                                               \
+   (void*)&sgpe::row_launch<R, N, E, RW, false>,                                                     \
+   (const void*)&sgpe::row_pass_kernel<R, N, E, RW, false>,                                          \
+   (void*)&sgpe::row_launch<R, N, E, RW, true>,                                                      \
+   (const void*)&sgpe::row_pass_kernel<R, N, E, RW, true>},
+#define SGPE_FROW_E(R, PR, N, E, RW)                                                                          \
+  {PR, N, E, RW, (void*)&sgpe::frow_launch<R, N, E, RW>, (const void*)&sgpe::fft_rows_kernel<R, N, E, RW>, \
+   nullptr, nullptr},
+#define SGPE_FCOL_E(R, PR, N, E, W)                                                                          \
+  {PR, N, E, W, (void*)&sgpe::fcol_launch<R, N, E, W>, (const void*)&sgpe::fft_cols_kernel<R, N, E, W>, \
+   nullptr, nullptr},
+#define SGPE_ECOL_E(R, PR, N, E, W)                                                                            \
+  {PR, N, E, W, (void*)&sgpe::ecol_launch<R, N, E, W>, (const void*)&sgpe::energy_col_kernel<R, N, E, W>, \
+   nullptr, nullptr},
+#define SGPE_EROW_E(R, PR, N, E, RW)                                                                             \
+  {PR, N, E, RW, (void*)&sgpe::erow_launch<R, N, E, RW>, (const void*)&sgpe::energy_row_kernel<R, N, E, RW>, \
+   nullptr, nullptr},
 
 // Tunable shapes of the two split-step passes: columns (N, E, W), rows (N, E, ROWS).
 #define SGPE_COL_SHAPES(X, R, PR)                                                                          \

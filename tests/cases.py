@@ -85,5 +85,18 @@ def tall(name='tall', seed=4):
     return ps, 'imag', 0.01
 
 
+def dense_kin(name='densekin', mesh=(64, 64), r=(8, 8), seed=5):
+    """Non-separable kinetic energy (a kx*ky cross term) with Raman coupling
+    in the rotated frame, imaginary time: the dense-kinetic kernels."""
+    ps = base(name, mesh, r, atoms=1e3)
+    ps.coupling_setup(wavel=WAVEL, scale=0.5, kin_shift=True)
+    ps.coupling_uniform(1.5)
+    kx, ky = ps.space['kx_mesh'], ps.space['ky_mesh']
+    ps.kin_eng_spin = [k + 0.05 * kx * ky for k in ps.kin_eng_spin]
+    ps.kin_eng_spin = [k - np.min(k) for k in ps.kin_eng_spin]
+    _noise(ps, seed)
+    return ps, 'imag', 5e-3
+
+
 CASES = {'ground': ground, 'raman_lab': raman_lab, 'raman_rot': raman_rot,
-         'dense_ops': dense_ops, 'tall': tall}
+         'dense_ops': dense_ops, 'dense_kin': dense_kin, 'tall': tall}

@@ -19,7 +19,7 @@ from spinor_gpe.pspinor.tensor_propagator import TensorPropagator
 pytestmark = pytest.mark.gpu
 
 TOL = {'c128': (1e-11, 1e-11), 'c64': (2e-4, 1e-5)}
-STEPS = {'ground': 60, 'raman_lab': 120, 'raman_rot': 60, 'dense_ops': 80, 'tall': 50}
+STEPS = {'ground': 60, 'raman_lab': 120, 'raman_rot': 60, 'dense_ops': 80, 'dense_kin': 60, 'tall': 50}
 
 
 def rel(a, b):

@@ -1,6 +1,8 @@
 """Every compiled FFT shape of both split-step passes gives the oracle's
 result (GPU): rows E x rows-per-block and columns E x columns-per-block for
-each grid length, complex128 and complex64."""
+each grid length, complex128 and complex64; the kernel variants for dense
+operators (non-separable kinetic energy and potential, a coupling array) on
+a subset of grid lengths."""
 
 import numpy as np
 import pytest
@@ -18,13 +20,30 @@ STEPS = 3
 _REF = {}
 
 
-def reference(n):
-    if n not in _REF:
-        ps, time, dt = raman_lab(name=f'shape{n}', mesh=(n, n), r=(24, 24))
+GENERAL_GRIDS = [64, 256, 1024]
+
+
+def make_case(n, general):
+    ps, time, dt = raman_lab(name=f'shape{n}{"g" if general else ""}', mesh=(n, n), r=(24, 24))
+    if general:
+        x, y = ps.space['x_mesh'], ps.space['y_mesh']
+        kx, ky = ps.space['kx_mesh'], ps.space['ky_mesh']
+        ps.pot_eng = ps.pot_eng + 2.0 * np.exp(-((x - 1.0) ** 2 + y ** 2) / 4)
+        ps.detuning_uniform(0.5 * ps.EL_recoil)
+        ps.coupling_grad(0.2, 2.0, axis=0)
+        ps.coupling = np.abs(ps.coupling)
+        ps.kin_eng_spin = [k + 0.02 * kx * ky for k in ps.kin_eng_spin]
+        ps.kin_eng_spin = [k - np.min(k) for k in ps.kin_eng_spin]
+    return ps, time, dt
+
+
+def reference(n, general=False):
+    if (n, general) not in _REF:
+        ps, time, dt = make_case(n, general)
         o = OraclePropagator(ps, dt, time=time)
         o.run(STEPS)
-        _REF[n] = (ps, time, dt, np.stack(o.psik_numpy()))
-    return _REF[n]
+        _REF[(n, general)] = (ps, time, dt, np.stack(o.psik_numpy()))
+    return _REF[(n, general)]
 
 
 def cases():
@@ -33,14 +52,17 @@ def cases():
         for prec in ('c128', 'c64'):
             for pass_ in (0, 1):
                 for shape in autotune.shapes(pass_, prec, n):
-                    out.append((n, prec, pass_, shape))
+                    out.append((n, prec, pass_, shape, False))
+                    if n in GENERAL_GRIDS:
+                        out.append((n, prec, pass_, shape, True))
     return out
 
 
-@pytest.mark.parametrize('n,prec,pass_,shape', cases())
-def test_shape_matches_oracle(gpu, n, prec, pass_, shape):
-    print(f"[shape] n={n} {prec} pass={'row' if pass_ == 0 else 'col'} shape={shape}", flush=True)
-    ps, time, dt, want = reference(n)
+@pytest.mark.parametrize('n,prec,pass_,shape,general', cases())
+def test_shape_matches_oracle(gpu, n, prec, pass_, shape, general):
+    print(f"[shape] n={n} {prec} pass={'row' if pass_ == 0 else 'col'} shape={shape} general={general}",
+          flush=True)
+    ps, time, dt, want = reference(n, general)
     spec = InstanceSpec.from_spin(ps, time, dt)
     plan = Plan(gpu, prec, n, n, 1)
     cur = plan.tuning()
