@@ -82,7 +82,18 @@ template <> __device__ __forceinline__ void sc<float>(float x, float* s, float* 
     return;
   }
   int q;
-  const float r = (float)cw_reduce((double)x, &q);
+  float r;
+  if (fabsf(x) <= 100.0f) {
+    // Single-precision reduction: pi/2 split into 8-, 12- and 24-bit parts,
+    // so k * part is exact for |k| <= 64 (within 2 ulp of sin/cos overall).
+    const float k = rintf(x * 0.63661977236758134f);
+    r = fmaf(-k, 1.5703125f, x);
+    r = fmaf(-k, 4.8387050628662109e-4f, r);
+    r = fmaf(-k, -4.3711388286737929e-8f, r);
+    q = (int)k & 3;
+  } else {
+    r = (float)cw_reduce((double)x, &q);
+  }
   const float r2 = r * r;
   float ps = 2.7557319223985893e-06f;
   ps = fmaf(ps, r2, -0.0001984126984126984f);
@@ -135,6 +146,18 @@ template <typename C> __device__ __forceinline__ void st_stream(C* p, C v) {
   typedef S V2 __attribute__((ext_vector_type(2)));
   const V2 w = {v.x, v.y};
   __builtin_nontemporal_store(w, reinterpret_cast<V2*>(p));
+}
+
+// Sum of n doubles by one wave: lane l adds entries l, l + 64, ..., then a
+// butterfly across the lanes; every lane gets the same total, and every wave
+// that calls it sums in the same order.
+__device__ __forceinline__ double wave_sum_strided(const double* __restrict__ p, int n) {
+  const int lane = threadIdx.x & 63;
+  double v = 0.0;
+  for (int i = lane; i < n; i += 64) v += p[i];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
 }
 
 template <typename R> __device__ __forceinline__ double norm2d(typename Cx<R>::T a) {
@@ -458,20 +481,16 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
 
   // Real-space normalization (tensor_tools.py:301-305) from the column
   // pass's k-space sums by Parseval: sum_r |IDFT2 v|^2 = Nx Ny sum_k |v|^2.
-  // Its loads join the ones above; the reduction's barriers publish the
-  // twiddles stored just before it.
-  double pv[2] = {0.0, 0.0}, ps[2] = {0.0, 0.0};
+  // Every wave sums all partials itself (same order in every wave and
+  // block: no barrier, deterministic); the loads join the ones above. The
+  // twiddles stored here are published by the transform's first barrier
+  // (its first stage uses none).
   const bool pops_block = first_sub && blockIdx.x == 0;
-  for (int i = threadIdx.x; i < nt2; i += blockDim.x) {
-    pv[0] += P.part[PART_NRM * nt2 + i];
-    if (pops_block) ps[i >= nt2 / 2 ? 1 : 0] += P.part[PART_S * nt2 + i];
-  }
+  double nrm[1] = {wave_sum_strided(P.part + PART_NRM * nt2, nt2)};
   const C* tw = stw.store(sm + ROWS * 2 * N);
-  double nrm[2];
-  block_sum<2>(pv, smem);
-  nrm[0] = pv[0];
   if (pops_block) {
-    block_sum<2>(ps, smem);
+    double ps[2] = {wave_sum_strided(P.part + PART_S * nt2, nt2 / 2),
+                    wave_sum_strided(P.part + PART_S * nt2 + nt2 / 2, nt2 / 2)};
     if (threadIdx.x == 0) {
       const int step = run.step;
       if (step > 0 && run.pops && step - 1 < run.pops_cap) {
