@@ -10,7 +10,7 @@ Differences from the reference, all deliberate:
 
 * a relative ``path`` is created under ``$SPINOR_GPE_DATA`` (default:
   ``./data``) instead of under the reference's source tree;
-* the plotting helpers (``plot_*``) are not provided;
+* the plotting helpers import matplotlib on first use;
 * ``imaginary``/``real`` accept keyword-only engine options (``precision``,
   ``block_steps``) after the reference's own parameters.
 """
@@ -22,6 +22,7 @@ import numpy as np
 from scipy.ndimage import fourier_shift
 
 from spinor_gpe import constants as const
+from spinor_gpe.pspinor import plotting_tools as ptools
 from spinor_gpe.pspinor import tensor_tools as ttools
 
 
@@ -356,6 +357,48 @@ class PSpinor:
                 winding = np.exp(windings[i, j] * 1j * np.arctan2(dy, dx))
                 self.psi[i] = self.psi[i] * core * winding
         self.psik = ttools.fft_2d(self.psi, delta_r=self.space['dr'])
+
+    def seed_regular_vortices(self):
+        """Not provided, as in the reference (pspinor.py:747-752)."""
+        raise NotImplementedError()
+
+    def seed_random_vortices(self, N):  # pylint: disable=invalid-name
+        """Not provided, as in the reference (pspinor.py:754-756)."""
+        raise NotImplementedError()
+
+    # ------------------------------------------------------------- figures
+    def _extent(self, key, scale):
+        """``[-x, x, -y, y] / scale`` from the half-widths ``space[key]``."""
+        half = np.asarray(self.space[key], dtype=float)
+        return np.array([-half[0], half[0], -half[1], half[1]]) / scale
+
+    def plot_rdens(self, psi=None, spin=None, cmap='viridis', scale=1.0):
+        """Real-space density of both components (``spin=None``) or of one
+        (0 up, 1 down), lengths in units of ``scale`` (pspinor.py:758-787)."""
+        return ptools.plot_dens(self.psi if psi is None else psi, spin, cmap, scale,
+                                extent=self._extent('r_sizes', scale))
+
+    def plot_kdens(self, psik=None, spin=None, cmap='viridis', scale=1.0):
+        """Momentum-space density, as :meth:`plot_rdens` (pspinor.py:789-818)."""
+        return ptools.plot_dens(self.psik if psik is None else psik, spin, cmap, scale,
+                                extent=self._extent('k_sizes', scale))
+
+    def plot_rphase(self, psi=None, spin=None, cmap='twilight_shifted', scale=1.0):
+        """Real-space phase, zero where the density is below 1e-6 of its
+        maximum (pspinor.py:820-851)."""
+        return ptools.plot_phase(self.psi if psi is None else psi, spin, cmap, scale,
+                                 extent=self._extent('r_sizes', scale))
+
+    # pylint: disable=too-many-arguments
+    def plot_spins(self, rscale=1.0, kscale=1.0, cmap='viridis', save=True, ext='.pdf',
+                   zoom=1.0):
+        """Densities (real and k-space) and phases of both components of the
+        current state (pspinor.py:853-887); saves
+        ``spin_dens_phase<i>-<folder><ext>`` in the data folder when ``save``.
+        Returns ``(fig, {'r', 'ph', 'k'})``."""
+        extents = {'r': self._extent('r_sizes', rscale), 'k': self._extent('k_sizes', kscale)}
+        return ptools.plot_spins(self.psi, self.psik, extents, self.paths, cmap=cmap,
+                                 save=save, ext=ext, zoom=zoom)
 
     # ---------------------------------------------------------- propagation
     def _propagate(self, time, t_step, n_steps, device, is_sampling,
