@@ -47,6 +47,16 @@ template <typename R> __device__ __forceinline__ void quadrant_fix(int q, R sr, 
 }
 
 template <typename R> __device__ __forceinline__ void sc(R x, R* s, R* c);
+// Taylor coefficients of sin(r)/r - 1 and cos(r) - 1 in powers of r^2,
+// highest first. Read from constant memory: the loads land in scalar
+// registers, where a fused multiply-add can take them directly, instead of
+// being rebuilt in vector registers at every use.
+inline __constant__ double kSinTaylor[8] = {2.8114572543455206e-15, -7.647163731819816e-13, 1.6059043836821613e-10,
+                                            -2.505210838544172e-08, 2.7557319223985893e-06, -0.0001984126984126984,
+                                            0.008333333333333333,   -0.16666666666666666};
+inline __constant__ double kCosTaylor[9] = {-1.5619206968586225e-16, 4.779477332387385e-14, -1.1470745597729725e-11,
+                                            2.08767569878681e-09,    -2.755731922398589e-07, 2.48015873015873e-05,
+                                            -0.001388888888888889,   0.041666666666666664,   -0.5};
 template <> __device__ __forceinline__ void sc<double>(double x, double* s, double* c) {
   if (!(fabs(x) <= 1.0e5)) {
     sincos(x, s, c);
@@ -55,24 +65,13 @@ template <> __device__ __forceinline__ void sc<double>(double x, double* s, doub
   int q;
   const double r = cw_reduce(x, &q);
   const double r2 = r * r;
-  double ps = 2.8114572543455206e-15;
-  ps = fma(ps, r2, -7.647163731819816e-13);
-  ps = fma(ps, r2, 1.6059043836821613e-10);
-  ps = fma(ps, r2, -2.505210838544172e-08);
-  ps = fma(ps, r2, 2.7557319223985893e-06);
-  ps = fma(ps, r2, -0.0001984126984126984);
-  ps = fma(ps, r2, 0.008333333333333333);
-  ps = fma(ps, r2, -0.16666666666666666);
+  double ps = kSinTaylor[0];
+#pragma unroll
+  for (int i = 1; i < 8; ++i) ps = fma(ps, r2, kSinTaylor[i]);
   const double sr = fma(r * r2, ps, r);
-  double pc = -1.5619206968586225e-16;
-  pc = fma(pc, r2, 4.779477332387385e-14);
-  pc = fma(pc, r2, -1.1470745597729725e-11);
-  pc = fma(pc, r2, 2.08767569878681e-09);
-  pc = fma(pc, r2, -2.755731922398589e-07);
-  pc = fma(pc, r2, 2.48015873015873e-05);
-  pc = fma(pc, r2, -0.001388888888888889);
-  pc = fma(pc, r2, 0.041666666666666664);
-  pc = fma(pc, r2, -0.5);
+  double pc = kCosTaylor[0];
+#pragma unroll
+  for (int i = 1; i < 9; ++i) pc = fma(pc, r2, kCosTaylor[i]);
   const double cr = fma(r2, pc, 1.0);
   quadrant_fix<double>(q, sr, cr, s, c);
 }
@@ -123,6 +122,16 @@ __device__ __forceinline__ typename Cx<R>::T phase_op(R e_dt) {
     sc<R>(-e_dt, &s, &c);
     return C{c, s};
   }
+}
+
+// p times a phase factor from phase_op: in imaginary time the factor is
+// real (zero imaginary part), and scaling by its real part gives the same
+// result as the complex product without its multiplications by zero.
+template <bool IMAG, typename C> __device__ __forceinline__ C mul_phase(C p, C u) {
+  if constexpr (IMAG)
+    return cscale(p, u.x);
+  else
+    return cmul(p, u);
 }
 
 // The pass kernels select their pointwise code by uniform flags (time mode,
@@ -228,10 +237,13 @@ __global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __re
   constexpr bool kKy = kTw && kData + 3 * (size_t)N * sizeof(C) <= kLdsBytes;
   extern __shared__ double smem[];
   const int inst = blockIdx.z;
-  if (!runs[inst].active) return;
-  const bool kspace = runs[inst].form == FORM_KSPACE;
   const InstDev& P = insts[inst];
+  // The state's address and row length are read together with the active
+  // flag, so the state loads wait for one round of scalar loads, not two.
+  C* const psi = reinterpret_cast<C*>(P.psi);
   const int nx = P.nx;
+  if (!runs[inst].active || !psi || nx <= 0) return;
+  const bool kspace = runs[inst].form == FORM_KSPACE;
   const int comp = blockIdx.y;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int c = threadIdx.x % W;
@@ -239,7 +251,7 @@ __global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __re
   const int jx = tile * W + c;
   C* const sm = reinterpret_cast<C*>(smem);
   C* lds = sm + c;
-  C* col = reinterpret_cast<C*>(P.psi) + (size_t)comp * N * nx + jx;
+  C* col = psi + (size_t)comp * N * nx + jx;
 
   // Every global load is issued before any computation: the state first,
   // then the kinetic tables and twiddles (staged into LDS).
@@ -349,8 +361,8 @@ __device__ __forceinline__ void row_pointwise(C (&a0)[E], C (&a1)[E], const Inst
     const R n1 = lam2r * (p1.x * p1.x + p1.y * p1.y);
     const C u0 = phase_op<R, IMAG>((guu * n0 + gud * n1) * hdt);
     const C u1 = phase_op<R, IMAG>((gdd * n1 + gud * n0) * hdt);
-    p0 = cscale(cmul(p0, u0), lam);
-    p1 = cscale(cmul(p1, u1), lam);
+    p0 = cscale(mul_phase<IMAG>(p0, u0), lam);
+    p1 = cscale(mul_phase<IMAG>(p1, u1), lam);
     R alpha = ca;
     C b01 = cb, b10 = cb;
     bool apply = true;
@@ -405,12 +417,28 @@ __device__ __forceinline__ void row_pointwise(C (&a0)[E], C (&a1)[E], const Inst
       p0 = s0;
       p1 = s1;
     }
-    a0[j] = cscale(cmul(p0, u0), cf);
-    a1[j] = cscale(cmul(p1, u1), cf);
+    a0[j] = cscale(mul_phase<IMAG>(p0, u0), cf);
+    a1[j] = cscale(mul_phase<IMAG>(p1, u1), cf);
   }
 }
 
 // ---------------------------------------------------------------- row pass
+
+// The x-table values (potential phases of both components, e^{-i theta})
+// of this thread's E points of a row; `stand_in` replaces an absent
+// potential table.
+template <int E, int T, typename C>
+__device__ __forceinline__ void load_x_tables(C (&tx0)[E], C (&tx1)[E], C (&tex)[E], const C* px0, const C* px1,
+                                              const C* eth, const C* stand_in, int t) {
+  const C* s0 = px0 ? px0 : stand_in;
+  const C* s1 = px1 ? px1 : stand_in;
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    tx0[j] = s0[t + j * T];
+    tx1[j] = s1[t + j * T];
+    tex[j] = eth[t + j * T];
+  }
+}
 
 // GENERAL: some slot has a dense potential or a coupling array; otherwise
 // every slot takes the separable fast path.
@@ -431,15 +459,18 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
   extern __shared__ double smem[];
   const int inst = blockIdx.z;
   RunDev& run = runs[inst];
-  if (!run.active) return;
   const InstDev& P = insts[inst];
+  // The state's address and size are read together with the active flag,
+  // so the state loads wait for one round of scalar loads, not two.
+  C* const psi = reinterpret_cast<C*>(P.psi);
   const int ny = P.ny;
+  if (!run.active || !psi || ny <= 0) return;
   const int nt2 = 2 * (N / col_w);
   const int row = threadIdx.x / T;
   const int t = threadIdx.x % T;
   const int y = blockIdx.x * ROWS + row;
   C* const sm = reinterpret_cast<C*>(smem);
-  C* r0 = reinterpret_cast<C*>(P.psi) + (size_t)y * N;
+  C* r0 = psi + (size_t)y * N;
   C* r1 = r0 + (size_t)ny * N;
   C* lds0 = sm + (size_t)row * 2 * N;
   C* lds1 = lds0 + N;
@@ -476,6 +507,9 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
     px1 = reinterpret_cast<const C*>(P.px[dur][1]);
   }
   const C* eth = reinterpret_cast<const C*>(P.etheta);
+  constexpr bool kEarlyTables = E <= 2;
+  C tx0[E], tx1[E], tex[E];
+  if constexpr (kEarlyTables) load_x_tables<E, T>(tx0, tx1, tex, px0, px1, eth, tws, t);
   TableStage<kTw, N, kBlock, C> stw;
   stw.load(tws);
 
@@ -486,11 +520,12 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
   // twiddles stored here are published by the transform's first barrier
   // (its first stage uses none).
   const bool pops_block = first_sub && blockIdx.x == 0;
-  double nrm[1] = {wave_sum_strided(P.part + PART_NRM * nt2, nt2)};
+  const double* part = P.part;
+  double nrm[1] = {wave_sum_strided(part + PART_NRM * nt2, nt2)};
   const C* tw = stw.store(sm + ROWS * 2 * N);
   if (pops_block) {
-    double ps[2] = {wave_sum_strided(P.part + PART_S * nt2, nt2 / 2),
-                    wave_sum_strided(P.part + PART_S * nt2 + nt2 / 2, nt2 / 2)};
+    double ps[2] = {wave_sum_strided(part + PART_S * nt2, nt2 / 2),
+                    wave_sum_strided(part + PART_S * nt2 + nt2 / 2, nt2 / 2)};
     if (threadIdx.x == 0) {
       const int step = run.step;
       if (step > 0 && run.pops && step - 1 < run.pops_cap) {
@@ -514,19 +549,11 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
     fft_run2<N, E, true, V>(a0, a1, t, tw, lds0, lds1);
 
   // The x-table values of this thread's E points are loaded together, so
-  // the block waits for memory once rather than once per point. An absent
-  // potential table is stood in for by the twiddles (N entries; unused).
-  C tx0[E], tx1[E], tex[E];
-  {
-    const C* s0 = px0 ? px0 : tw;
-    const C* s1 = px1 ? px1 : tw;
-#pragma unroll
-    for (int j = 0; j < E; ++j) {
-      tx0[j] = s0[t + j * T];
-      tx1[j] = s1[t + j * T];
-      tex[j] = eth[t + j * T];
-    }
-  }
+  // the block waits for memory once rather than once per point: with the
+  // state at the start when E <= 2 (few registers to hold them through the
+  // inverse transform), otherwise after it. An absent potential table is
+  // stood in for by the global twiddle table (N entries; unused).
+  if constexpr (!kEarlyTables) load_x_tables<E, T>(tx0, tx1, tex, px0, px1, eth, tws, t);
   if (P.imag)
     row_pointwise<R, N, E, T, true, !GENERAL>(a0, a1, P, dur, y, t, lam2, py0, py1, tx0, tx1, tex);
   else

@@ -18,9 +18,13 @@ OUT = os.path.join(PKG, '_lib', 'libsgpe.so')
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 ARCH = os.environ.get('SGPE_ARCH', 'gfx950')
 # Packed FP32 instructions are disabled: some of their forms have produced
-# silently wrong results on this hardware pool.
+# silently wrong results on this hardware. Kernel arguments are preloaded
+# into scalar registers at wave launch (gfx950 supports it; the code object
+# keeps a fallback entry that loads them from memory), which takes one
+# dependent memory round trip off the start of every kernel.
 FLAGS = ['-O3', '-std=c++17', f'--offload-arch={ARCH}', '-fPIC', '-Wno-unused-result',
-         '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops']
+         '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
+         '-mllvm', '-amdgpu-kernarg-preload-count=16']
 SOURCES = ['sgpe_api.hip', 'sgpe_inst_f64.hip', 'sgpe_inst_f32.hip', 'sgpe_unwrap.cpp']
 
 
