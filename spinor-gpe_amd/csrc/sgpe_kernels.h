@@ -47,16 +47,6 @@ template <typename R> __device__ __forceinline__ void quadrant_fix(int q, R sr, 
 }
 
 template <typename R> __device__ __forceinline__ void sc(R x, R* s, R* c);
-// Taylor coefficients of sin(r)/r - 1 and cos(r) - 1 in powers of r^2,
-// highest first. Read from constant memory: the loads land in scalar
-// registers, where a fused multiply-add can take them directly, instead of
-// being rebuilt in vector registers at every use.
-inline __constant__ double kSinTaylor[8] = {2.8114572543455206e-15, -7.647163731819816e-13, 1.6059043836821613e-10,
-                                            -2.505210838544172e-08, 2.7557319223985893e-06, -0.0001984126984126984,
-                                            0.008333333333333333,   -0.16666666666666666};
-inline __constant__ double kCosTaylor[9] = {-1.5619206968586225e-16, 4.779477332387385e-14, -1.1470745597729725e-11,
-                                            2.08767569878681e-09,    -2.755731922398589e-07, 2.48015873015873e-05,
-                                            -0.001388888888888889,   0.041666666666666664,   -0.5};
 template <> __device__ __forceinline__ void sc<double>(double x, double* s, double* c) {
   if (!(fabs(x) <= 1.0e5)) {
     sincos(x, s, c);
@@ -65,13 +55,24 @@ template <> __device__ __forceinline__ void sc<double>(double x, double* s, doub
   int q;
   const double r = cw_reduce(x, &q);
   const double r2 = r * r;
-  double ps = kSinTaylor[0];
-#pragma unroll
-  for (int i = 1; i < 8; ++i) ps = fma(ps, r2, kSinTaylor[i]);
+  double ps = 2.8114572543455206e-15;
+  ps = fma(ps, r2, -7.647163731819816e-13);
+  ps = fma(ps, r2, 1.6059043836821613e-10);
+  ps = fma(ps, r2, -2.505210838544172e-08);
+  ps = fma(ps, r2, 2.7557319223985893e-06);
+  ps = fma(ps, r2, -0.0001984126984126984);
+  ps = fma(ps, r2, 0.008333333333333333);
+  ps = fma(ps, r2, -0.16666666666666666);
   const double sr = fma(r * r2, ps, r);
-  double pc = kCosTaylor[0];
-#pragma unroll
-  for (int i = 1; i < 9; ++i) pc = fma(pc, r2, kCosTaylor[i]);
+  double pc = -1.5619206968586225e-16;
+  pc = fma(pc, r2, 4.779477332387385e-14);
+  pc = fma(pc, r2, -1.1470745597729725e-11);
+  pc = fma(pc, r2, 2.08767569878681e-09);
+  pc = fma(pc, r2, -2.755731922398589e-07);
+  pc = fma(pc, r2, 2.48015873015873e-05);
+  pc = fma(pc, r2, -0.001388888888888889);
+  pc = fma(pc, r2, 0.041666666666666664);
+  pc = fma(pc, r2, -0.5);
   const double cr = fma(r2, pc, 1.0);
   quadrant_fix<double>(q, sr, cr, s, c);
 }
