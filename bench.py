@@ -36,6 +36,7 @@ for _p in (os.path.join(ROOT, 'spinor-gpe_amd'), ROOT):
 import torch  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+BLOCK_REPS = 20  # graph blocks timed one by one for the median and MAD
 TRAFFIC_FILE = 'profiles/hbm_traffic_512x512.json'
 TITAN_V_512 = 149.1  # split-steps/s, bench_data/TitanV_cuda_step.npz (512^2, complex128)
 METRIC = ("split-steps/s at 512² (1 GPU); sweep solves/s + p50/p99 latency at "
@@ -71,12 +72,25 @@ def time_engine(ps, device, precision, steps, warmup, block, prof_steps, tune):
     plan.run(steps, block=block)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    # Per-block spread, as the reference reports median and MAD of repeated
+    # timings (benchmark_prop.py:92-93): BLOCK_REPS more graph blocks, each
+    # bracketed by events on the launch stream.
+    events = [torch.cuda.Event(enable_timing=True) for _ in range(BLOCK_REPS + 1)]
+    events[0].record()
+    for i in range(BLOCK_REPS):
+        plan.run(block, block=block)
+        events[i + 1].record()
+    torch.cuda.synchronize()
+    per_step = np.array([events[i].elapsed_time(events[i + 1]) / block for i in range(BLOCK_REPS)])
+    med = float(np.median(per_step))
+    blocks = {'blocks': BLOCK_REPS, 'steps_per_block': block, 'median_ms_per_step': round(med, 5),
+              'mad_ms_per_step': round(float(np.median(np.abs(per_step - med))), 6)}
     col_ms, row_ms, n_col, n_row = plan.profile_steps(prof_steps)
     plan.sync()
     st = plan.states[0]
     atoms = float(torch.sum(torch.abs(st) ** 2).item()) * ps.space['dv_k']
     return {'elapsed': elapsed, 'col_ms': col_ms / n_col, 'row_ms': row_ms / n_row,
-            'tuning': plan.tuning(), 'streaming': plan.streaming(), 'atoms': atoms}
+            'tuning': plan.tuning(), 'streaming': plan.streaming(), 'atoms': atoms, 'blocks': blocks}
 
 
 def sweep_leg(device, check_every, fp64_sample):
@@ -256,13 +270,15 @@ def step_leg(args, device):
                    'baseline': 'Titan V 149.1 split-steps/s at 512^2, complex128 '
                                '(reference bench_data/TitanV_cuda_step.npz)'},
         'grid_point_steps_per_s': round(sps * n_pts, 1),
+        'block_timing': r['blocks'],
         'roofline': roofline(r, 'c128'),
         'atoms_after_run': {p: res['atoms'] for p, res in results.items()},
     }
     if r64:
         sps64 = args.steps / r64['elapsed']
         line['c64'] = {'value': round(sps64, 2), 'ms_per_step': round(r64['elapsed'] / args.steps * 1e3, 5),
-                       'grid_point_steps_per_s': round(sps64 * n_pts, 1), 'roofline': roofline(r64, 'c64'),
+                       'grid_point_steps_per_s': round(sps64 * n_pts, 1), 'block_timing': r64['blocks'],
+                       'roofline': roofline(r64, 'c64'),
                        'dtype': 'c64 (f32 arithmetic, f64 reductions)'}
     if not args.no_cpu_baseline:
         log(f"CPU baseline: oracle full_step() for ~{args.cpu_budget:.0f} s")

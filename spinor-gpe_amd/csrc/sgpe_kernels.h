@@ -449,7 +449,7 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
                                                                 const typename Cx<R>::T* __restrict__ tws,
                                                                 int dur, int first_sub, int col_w, int stream) {
   typedef typename Cx<R>::T C;
-  typedef LdsView<1, 1 << 30, 0> V;
+  typedef SwzView<sizeof(C)> V;
   constexpr int T = N / E;
   // LDS: [ROWS*2*N exchange][N twiddles][ROWS*2*N second exchange buffer,
   // when double-buffered]. The x phase tables stay in global memory (cache
