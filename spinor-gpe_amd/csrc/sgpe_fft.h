@@ -262,13 +262,15 @@ template <int N, int E, int L = 1> constexpr int fft_exchanges() {
     return 1 + fft_exchanges<N, E, L * R>();
 }
 
-// Whether a row pass double-buffers its exchanges: for small radices (many
+// Whether a row pass double-buffers its exchanges: for radix 2 only (many
 // stages, where the saved barriers matter) and only while the second buffer
-// keeps the kernel within 64 KiB of LDS (more would cost occupancy).
-// Same predicate on host and device.
+// keeps the kernel within 64 KiB of LDS. Radix-4 row passes run batched
+// plans, where the second buffer's LDS would cap the workgroups per CU
+// below what their registers allow (complex128: 6 instead of 4). Same
+// predicate on host and device.
 constexpr size_t kDbLdsBytes = 64 * 1024;
 constexpr bool row_double_buffer(int e, size_t data_bytes, size_t other_bytes) {
-  return e <= 4 && 2 * data_bytes + other_bytes <= kDbLdsBytes;
+  return e <= 2 && 2 * data_bytes + other_bytes <= kDbLdsBytes;
 }
 
 // Twiddle table for the FFT stages: copies the n - 1 stage-major entries to

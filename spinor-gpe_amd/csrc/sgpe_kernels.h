@@ -330,13 +330,13 @@ __device__ __forceinline__ void part_sums(const double* part, int nt2, int slot,
   out2[1] = v[1];
 }
 
-// Real-space block of the row pass on this thread's E points of row y, both
+// Real-space block of the row pass on this thread's points J0 <= j < J1 of row y, both
 // components: U C P C U with the normalization folded in (tensor_propagator.
 // py:244-267). FAST: separable potential (x and y tables) and no or uniform
 // coupling, which is then always applied (the identity when off) with its
 // e^{-i theta} table (ones in the rotated frame); otherwise dense potentials
 // and coupling arrays where present.
-template <typename R, int N, int E, int T, bool IMAG, bool FAST, typename C>
+template <typename R, int N, int E, int T, bool IMAG, bool FAST, int J0 = 0, int J1 = E, typename C>
 __device__ __forceinline__ void row_pointwise(C (&a0)[E], C (&a1)[E], const InstDev& P, int dur, int y, int t,
                                               double lam2, C py0, C py1, const C (&tx0)[E], const C (&tx1)[E],
                                               const C (&tex)[E]) {
@@ -353,7 +353,7 @@ __device__ __forceinline__ void row_pointwise(C (&a0)[E], C (&a1)[E], const Inst
   const double* vd0 = P.v_dense[0];
   const double* vd1 = P.v_dense[1];
 #pragma unroll
-  for (int j = 0; j < E; ++j) {
+  for (int j = J0; j < J1; ++j) {
     const int x = t + j * T;
     C p0 = a0[j], p1 = a1[j];
     // Interaction half step U_s = exp(-i (g n) Delta/2) from the normalized
@@ -426,15 +426,15 @@ __device__ __forceinline__ void row_pointwise(C (&a0)[E], C (&a1)[E], const Inst
 // ---------------------------------------------------------------- row pass
 
 // The x-table values (potential phases of both components, e^{-i theta})
-// of this thread's E points of a row; `stand_in` replaces an absent
-// potential table.
-template <int E, int T, typename C>
+// of this thread's points J0 <= j < J1 of a row; `stand_in` replaces an
+// absent potential table.
+template <int E, int T, int J0 = 0, int J1 = E, typename C>
 __device__ __forceinline__ void load_x_tables(C (&tx0)[E], C (&tx1)[E], C (&tex)[E], const C* px0, const C* px1,
                                               const C* eth, const C* stand_in, int t) {
   const C* s0 = px0 ? px0 : stand_in;
   const C* s1 = px1 ? px1 : stand_in;
 #pragma unroll
-  for (int j = 0; j < E; ++j) {
+  for (int j = J0; j < J1; ++j) {
     tx0[j] = s0[t + j * T];
     tx1[j] = s1[t + j * T];
     tex[j] = eth[t + j * T];
@@ -549,16 +549,31 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
   else
     fft_run2<N, E, true, V>(a0, a1, t, tw, lds0, lds1);
 
-  // The x-table values of this thread's E points are loaded together, so
-  // the block waits for memory once rather than once per point: with the
-  // state at the start when E <= 2 (few registers to hold them through the
-  // inverse transform), otherwise after it. An absent potential table is
-  // stood in for by the global twiddle table (N entries; unused).
-  if constexpr (!kEarlyTables) load_x_tables<E, T>(tx0, tx1, tex, px0, px1, eth, tws, t);
+  // The x-table values of this thread's points are loaded together, so the
+  // block waits for memory once per group rather than once per point: with
+  // the state at the start when E <= 2 (few registers to hold them through
+  // the inverse transform), otherwise after it, in two halves. An absent
+  // potential table is stood in for by the global twiddle table (N entries;
+  // unused).
+  // Radix >= 4: the tables of the second half of the points are loaded only
+  // after the first half is done (the scheduling barrier keeps the compiler
+  // from hoisting those loads), so at most half of them are live at once:
+  // 216 -> 162 VGPRs for the complex128 radix-4 pass (3 waves per SIMD
+  // instead of 2), paid for with one more L2 round trip per row.
+  constexpr int H = kEarlyTables ? E : E / 2;
+  if constexpr (!kEarlyTables) load_x_tables<E, T, 0, H>(tx0, tx1, tex, px0, px1, eth, tws, t);
   if (P.imag)
-    row_pointwise<R, N, E, T, true, !GENERAL>(a0, a1, P, dur, y, t, lam2, py0, py1, tx0, tx1, tex);
+    row_pointwise<R, N, E, T, true, !GENERAL, 0, H>(a0, a1, P, dur, y, t, lam2, py0, py1, tx0, tx1, tex);
   else
-    row_pointwise<R, N, E, T, false, !GENERAL>(a0, a1, P, dur, y, t, lam2, py0, py1, tx0, tx1, tex);
+    row_pointwise<R, N, E, T, false, !GENERAL, 0, H>(a0, a1, P, dur, y, t, lam2, py0, py1, tx0, tx1, tex);
+  if constexpr (H < E) {
+    __builtin_amdgcn_sched_barrier(0);
+    load_x_tables<E, T, H, E>(tx0, tx1, tex, px0, px1, eth, tws, t);
+    if (P.imag)
+      row_pointwise<R, N, E, T, true, !GENERAL, H, E>(a0, a1, P, dur, y, t, lam2, py0, py1, tx0, tx1, tex);
+    else
+      row_pointwise<R, N, E, T, false, !GENERAL, H, E>(a0, a1, P, dur, y, t, lam2, py0, py1, tx0, tx1, tex);
+  }
   // The forward transform starts on the buffer pair the inverse one read
   // last but one, so no barrier is needed between them.
   if constexpr (kDb && fft_exchanges<N, E>() % 2)
