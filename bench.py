@@ -14,14 +14,19 @@ precision); the complex64 rate is reported under ``"c64"``. ``roofline``
 prices the dominant kernel's algorithmic bytes (read + write of the state,
 2 * 2 * Nx * Ny * 16 B) against the 8 TB/s HBM peak, with per-kernel average
 durations from HIP events on the launch stream; ``traffic`` is that kernel's
-measured HBM bytes per dispatch (rocprofv3 counters, profiles folder). ``cpu_baseline`` times the
-CPU oracle (an op-for-op torch restatement of the reference's path) on a
-bounded sample of the same workload.
+measured HBM bytes per dispatch (rocprofv3 counters, profiles folder).
+``cpu_baseline`` times the CPU oracle (an op-for-op torch restatement of the
+reference's path) on a bounded sample of the same workload (and of the 256^2
+ground-state workload). Further legs, reported beside the metric: ``large``
+(the 2048^2 vortex workload, where the passes are bandwidth-bound),
+``sweep`` (96 batched ground states) and ``serve`` (a continuous-batching
+request stream).
 """
 
 import argparse
 import json
 import os
+import platform
 import sys
 import tempfile
 import time
@@ -37,7 +42,7 @@ import torch  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 BLOCK_REPS = 20  # graph blocks timed one by one for the median and MAD
-TRAFFIC_FILE = 'profiles/hbm_traffic_512x512.json'
+TRAFFIC_FILE = 'profiles/hbm_traffic_pass_kernels.json'
 TITAN_V_512 = 149.1  # split-steps/s, bench_data/TitanV_cuda_step.npz (512^2, complex128)
 METRIC = ("split-steps/s at 512² (1 GPU); sweep solves/s + p50/p99 latency at "
           "ΔE≤1e-6")
@@ -56,14 +61,15 @@ def c1_spinor(device):
     return wl.c1_spinor(res.psi, ps0.space, 'c1')
 
 
-def time_engine(ps, device, precision, steps, warmup, block, prof_steps, tune):
+def time_engine(ps, device, precision, steps, warmup, block, prof_steps, tune, dt=1e-3):
     from spinor_gpe.engine import autotune
     from spinor_gpe.pspinor.tensor_propagator import TensorPropagator
+    ny, nx = ps.psi[0].shape
     if tune:
-        shape, timings = autotune.tune(torch.device(device), precision, 512, 512, 1)
-        autotune.store(precision, 512, 512, 1, shape, timings)
+        shape, timings = autotune.tune(torch.device(device), precision, ny, nx, 1)
+        autotune.store(precision, ny, nx, 1, shape, timings)
         log(f"tuned {precision}: {shape} {timings}")
-    prop = TensorPropagator(ps, 1e-3, steps, device, time='real', precision=precision,
+    prop = TensorPropagator(ps, dt, steps, device, time='real', precision=precision,
                             block_steps=block, progress=False)
     plan = prop._plan  # pylint: disable=protected-access
     plan.run(warmup, block=block)
@@ -204,10 +210,22 @@ def measured_traffic(kernel):
     return entry['traffic_bytes'] if entry else None
 
 
-def cpu_baseline(ps, budget_s):
-    """Time the CPU oracle's full_step() on the same workload (bounded sample)."""
+def cpu_model():
+    """The host CPU's model name (first entry of /proc/cpuinfo)."""
+    try:
+        with open('/proc/cpuinfo', encoding='utf8') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or 'unknown'
+
+
+def time_oracle(ps, dt, time_mode, budget_s):
+    """Median seconds of the CPU oracle's full_step() over ~budget_s."""
     from oracle.reference_path import OraclePropagator
-    prop = OraclePropagator(ps, 1e-3, time='real')
+    prop = OraclePropagator(ps, dt, time=time_mode)
     prop.full_step()
     times = []
     t_start = time.perf_counter()
@@ -215,11 +233,66 @@ def cpu_baseline(ps, budget_s):
         t0 = time.perf_counter()
         prop.full_step()
         times.append(time.perf_counter() - t0)
-    med = float(np.median(times))
+    return float(np.median(times)), len(times)
+
+
+def cpu_baseline(ps, budget_s):
+    """Time the CPU oracle's full_step() on the same workload (bounded
+    sample), and on the 256^2 ground-state workload for a third of that."""
+    from spinor_gpe.engine import workloads as wl
+    med, n = time_oracle(ps, 1e-3, 'real', budget_s)
+    med256, n256 = time_oracle(wl.c0_spinor('c0cpu'), 0.02, 'imag', budget_s / 3)
     return {'value': 1.0 / med, 'unit': 'split-steps/s', 'cores': torch.get_num_threads(),
-            'kind': 'port',
-            'sample': (f"512^2 Raman workload, complex128, {len(times)} timed full_step() calls "
-                       f"of the torch-CPU oracle (median {med * 1e3:.1f} ms)")}
+            'kind': 'port', 'cpu_model': cpu_model(),
+            'sample': (f"512^2 Raman workload, complex128, {n} timed full_step() calls "
+                       f"of the torch-CPU oracle (median {med * 1e3:.1f} ms)"),
+            'at_256': {'value': 1.0 / med256, 'unit': 'split-steps/s',
+                       'sample': (f"256^2 imaginary-time ground-state workload, complex128, {n256} "
+                                  f"timed full_step() calls (median {med256 * 1e3:.1f} ms)")}}
+
+
+def roofline(res, prec, n):
+    """HBM roofline of the slower pass of an n x n single-instance run."""
+    dom = 'row_pass' if res['row_ms'] >= res['col_ms'] else 'col_pass'
+    ms = max(res['row_ms'], res['col_ms'])
+    alg = 2 * 2 * n * n * {'c128': 16, 'c64': 8}[prec]
+    achieved = alg / (ms * 1e-3) / 1e9
+    row_e, row_rows, col_e, col_w = res['tuning']
+    real = 'double' if prec == 'c128' else 'float'
+    kname = (f"row_pass_kernel<{real}, {n}, {row_e}, {row_rows}>" if dom == 'row_pass'
+             else f"col_pass_kernel<{real}, {n}, {col_e}, {col_w}>")
+    if res['streaming'][0 if dom == 'row_pass' else 1]:
+        kname += ' streaming'
+    traffic = measured_traffic(kname)
+    return {'bound': 'hbm', 'kernel': dom, 'achieved': round(achieved, 1),
+            'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4),
+            'traffic': traffic, 'traffic_source': TRAFFIC_FILE if traffic else None,
+            'kernel_instance': kname, 'avg_us': round(ms * 1e3, 3), 'algorithmic_bytes': alg,
+            'col_pass_us': round(res['col_ms'] * 1e3, 3), 'row_pass_us': round(res['row_ms'] * 1e3, 3)}
+
+
+def large_leg(args, device):
+    """The 2048^2 workload (real time, dt = 5e-4, Raman kL = 3 with Omega =
+    1.5 E_R in the lab frame, 1e6 atoms, 64 seeded vortices on component 0;
+    workloads.c4_spinor) in complex128 and complex64: one instance large
+    enough that the passes are bound by bandwidth rather than latency."""
+    from spinor_gpe.engine import workloads as wl
+    log("building the 2048^2 vortex workload")
+    ps = wl.c4_spinor('c4bench')
+    n = 2048
+    out = {'workload': '2048x2048 real-time Raman-coupled GPE (kL=3, Omega=1.5 E_R, lab frame), '
+                       '64 vortices on component 0, 1e6 atoms, dt=5e-4, 1 instance'}
+    for prec in args.precisions:
+        log(f"timing 2048^2 {prec}: warmup {args.large_warmup}, steps {args.large_steps}")
+        r = time_engine(ps, device, prec, args.large_steps, args.large_warmup, args.block,
+                        args.profile_steps // 4, False, dt=5e-4)
+        sps = args.large_steps / r['elapsed']
+        out[prec] = {'value': round(sps, 2), 'unit': 'split-steps/s',
+                     'ms_per_step': round(r['elapsed'] / args.large_steps * 1e3, 5),
+                     'grid_point_steps_per_s': round(sps * n * n, 1), 'block_timing': r['blocks'],
+                     'tuning_row_e_rows_col_e_w': list(r['tuning']), 'streaming_row_col': list(r['streaming']),
+                     'roofline': roofline(r, prec, n), 'atoms_after_run': r['atoms']}
+    return out
 
 
 def step_leg(args, device):
@@ -234,26 +307,6 @@ def step_leg(args, device):
                                     args.profile_steps, args.tune)
     r = results['c128']
     sps = args.steps / r['elapsed']
-    bytes_c = {'c128': 16, 'c64': 8}
-
-    def roofline(res, prec):
-        dom = 'row_pass' if res['row_ms'] >= res['col_ms'] else 'col_pass'
-        ms = max(res['row_ms'], res['col_ms'])
-        alg = 2 * 2 * n_pts * bytes_c[prec]
-        achieved = alg / (ms * 1e-3) / 1e9
-        row_e, row_rows, col_e, col_w = res['tuning']
-        real = 'double' if prec == 'c128' else 'float'
-        kname = (f"row_pass_kernel<{real}, 512, {row_e}, {row_rows}>" if dom == 'row_pass'
-                 else f"col_pass_kernel<{real}, 512, {col_e}, {col_w}>")
-        if res['streaming'][0 if dom == 'row_pass' else 1]:
-            kname += ' streaming'
-        traffic = measured_traffic(kname)
-        return {'bound': 'hbm', 'kernel': dom, 'achieved': round(achieved, 1),
-                'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4),
-                'traffic': traffic, 'traffic_source': TRAFFIC_FILE if traffic else None,
-                'kernel_instance': kname, 'avg_us': round(ms * 1e3, 3), 'algorithmic_bytes': alg,
-                'col_pass_us': round(res['col_ms'] * 1e3, 3), 'row_pass_us': round(res['row_ms'] * 1e3, 3)}
-
     r64 = results.get('c64')
     line = {
         'metric': METRIC, 'value': round(sps, 2), 'unit': 'split-steps/s', 'n_gpus': 1,
@@ -271,14 +324,14 @@ def step_leg(args, device):
                                '(reference bench_data/TitanV_cuda_step.npz)'},
         'grid_point_steps_per_s': round(sps * n_pts, 1),
         'block_timing': r['blocks'],
-        'roofline': roofline(r, 'c128'),
+        'roofline': roofline(r, 'c128', 512),
         'atoms_after_run': {p: res['atoms'] for p, res in results.items()},
     }
     if r64:
         sps64 = args.steps / r64['elapsed']
         line['c64'] = {'value': round(sps64, 2), 'ms_per_step': round(r64['elapsed'] / args.steps * 1e3, 5),
                        'grid_point_steps_per_s': round(sps64 * n_pts, 1), 'block_timing': r64['blocks'],
-                       'roofline': roofline(r64, 'c64'),
+                       'roofline': roofline(r64, 'c64', 512),
                        'dtype': 'c64 (f32 arithmetic, f64 reductions)'}
     if not args.no_cpu_baseline:
         log(f"CPU baseline: oracle full_step() for ~{args.cpu_budget:.0f} s")
@@ -298,10 +351,13 @@ def main():
     ap.add_argument('--tune', action='store_true',
                     help='re-measure the FFT shapes of the workload instead of using the shipped table')
     ap.add_argument('--precisions', nargs='+', default=['c128', 'c64'], choices=['c128', 'c64'])
-    ap.add_argument('--legs', nargs='+', default=['step', 'sweep', 'serve'],
-                    choices=['step', 'sweep', 'serve'],
-                    help='step: 512^2 split-step rate (the metric value); sweep: batched ground '
-                         'states; serve: continuous-batching request stream')
+    ap.add_argument('--legs', nargs='+', default=['step', 'large', 'sweep', 'serve'],
+                    choices=['step', 'large', 'sweep', 'serve'],
+                    help='step: 512^2 split-step rate (the metric value); large: the 2048^2 '
+                         'workload; sweep: batched ground states; serve: continuous-batching '
+                         'request stream')
+    ap.add_argument('--large-steps', type=int, default=1000)
+    ap.add_argument('--large-warmup', type=int, default=100)
     ap.add_argument('--requests', type=int, default=48, help='requests of the served stream')
     ap.add_argument('--load', type=float, default=0.8, help='arrival rate / measured capacity')
     ap.add_argument('--sweep-check', type=int, default=100, help='full steps between convergence checks')
@@ -330,6 +386,8 @@ def main():
         log(f"calibration copy: {args.calibrate_copy} MiB read + {args.calibrate_copy} MiB written")
         del src, dst
     line = step_leg(args, device) if 'step' in args.legs else {'metric': METRIC, 'value': None}
+    if 'large' in args.legs:
+        line['large'] = large_leg(args, device)
     if 'sweep' in args.legs:
         line['sweep'] = sweep_leg(device, args.sweep_check, args.fp64_sample)
     if 'serve' in args.legs:
