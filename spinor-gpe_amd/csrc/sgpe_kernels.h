@@ -441,6 +441,25 @@ __device__ __forceinline__ void load_x_tables(C (&tx0)[E], C (&tx1)[E], C (&tex)
   }
 }
 
+// The row pointwise block point by point from point J on, point J's tables
+// already loaded: point J + 1's tables are loaded first, so their latency
+// overlaps point J's arithmetic, and at most two points' tables are live.
+template <typename R, int N, int E, int T, bool FAST, int J, typename C>
+__device__ __forceinline__ void pointwise_points(C (&a0)[E], C (&a1)[E], const InstDev& P, int dur, int y, int t,
+                                                 double lam2, C py0, C py1, C (&tx0)[E], C (&tx1)[E], C (&tex)[E],
+                                                 const C* px0, const C* px1, const C* eth, const C* tws) {
+  if constexpr (J < E) {
+    __builtin_amdgcn_sched_barrier(0);
+    if constexpr (J + 1 < E) load_x_tables<E, T, J + 1, J + 2>(tx0, tx1, tex, px0, px1, eth, tws, t);
+    if (P.imag)
+      row_pointwise<R, N, E, T, true, FAST, J, J + 1>(a0, a1, P, dur, y, t, lam2, py0, py1, tx0, tx1, tex);
+    else
+      row_pointwise<R, N, E, T, false, FAST, J, J + 1>(a0, a1, P, dur, y, t, lam2, py0, py1, tx0, tx1, tex);
+    pointwise_points<R, N, E, T, FAST, J + 1>(a0, a1, P, dur, y, t, lam2, py0, py1, tx0, tx1, tex, px0, px1, eth,
+                                              tws);
+  }
+}
+
 // GENERAL: some slot has a dense potential or a coupling array; otherwise
 // every slot takes the separable fast path.
 template <typename R, int N, int E, int ROWS, bool GENERAL>
@@ -555,24 +574,35 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
   // the inverse transform), otherwise after it, in two halves. An absent
   // potential table is stood in for by the global twiddle table (N entries;
   // unused).
-  // Radix >= 4: the tables of the second half of the points are loaded only
+  // Radix 4: the tables of the second half of the points are loaded only
   // after the first half is done (the scheduling barrier keeps the compiler
   // from hoisting those loads), so at most half of them are live at once:
   // 216 -> 162 VGPRs for the complex128 radix-4 pass (3 waves per SIMD
-  // instead of 2), paid for with one more L2 round trip per row.
-  constexpr int H = kEarlyTables ? E : E / 2;
-  if constexpr (!kEarlyTables) load_x_tables<E, T, 0, H>(tx0, tx1, tex, px0, px1, eth, tws, t);
-  if (P.imag)
-    row_pointwise<R, N, E, T, true, !GENERAL, 0, H>(a0, a1, P, dur, y, t, lam2, py0, py1, tx0, tx1, tex);
-  else
-    row_pointwise<R, N, E, T, false, !GENERAL, 0, H>(a0, a1, P, dur, y, t, lam2, py0, py1, tx0, tx1, tex);
-  if constexpr (H < E) {
-    __builtin_amdgcn_sched_barrier(0);
-    load_x_tables<E, T, H, E>(tx0, tx1, tex, px0, px1, eth, tws, t);
+  // instead of 2), paid for with one more L2 round trip per row. Complex128
+  // radix >= 8: point by point, each point's tables loaded while the
+  // previous point is computed (296 -> 224 VGPRs, 2 waves per SIMD; with the
+  // twiddles in global memory two 2048-point workgroups share a CU). In
+  // complex64 the halves already fit 3 waves per SIMD, and the point-by-
+  // point loads only add latency.
+  if constexpr (E >= 8 && sizeof(C) == 16) {
+    load_x_tables<E, T, 0, 1>(tx0, tx1, tex, px0, px1, eth, tws, t);
+    pointwise_points<R, N, E, T, !GENERAL, 0>(a0, a1, P, dur, y, t, lam2, py0, py1, tx0, tx1, tex, px0, px1, eth,
+                                              tws);
+  } else {
+    constexpr int H = kEarlyTables ? E : E / 2;
+    if constexpr (!kEarlyTables) load_x_tables<E, T, 0, H>(tx0, tx1, tex, px0, px1, eth, tws, t);
     if (P.imag)
-      row_pointwise<R, N, E, T, true, !GENERAL, H, E>(a0, a1, P, dur, y, t, lam2, py0, py1, tx0, tx1, tex);
+      row_pointwise<R, N, E, T, true, !GENERAL, 0, H>(a0, a1, P, dur, y, t, lam2, py0, py1, tx0, tx1, tex);
     else
-      row_pointwise<R, N, E, T, false, !GENERAL, H, E>(a0, a1, P, dur, y, t, lam2, py0, py1, tx0, tx1, tex);
+      row_pointwise<R, N, E, T, false, !GENERAL, 0, H>(a0, a1, P, dur, y, t, lam2, py0, py1, tx0, tx1, tex);
+    if constexpr (H < E) {
+      __builtin_amdgcn_sched_barrier(0);
+      load_x_tables<E, T, H, E>(tx0, tx1, tex, px0, px1, eth, tws, t);
+      if (P.imag)
+        row_pointwise<R, N, E, T, true, !GENERAL, H, E>(a0, a1, P, dur, y, t, lam2, py0, py1, tx0, tx1, tex);
+      else
+        row_pointwise<R, N, E, T, false, !GENERAL, H, E>(a0, a1, P, dur, y, t, lam2, py0, py1, tx0, tx1, tex);
+    }
   }
   // The forward transform starts on the buffer pair the inverse one read
   // last but one, so no barrier is needed between them.
