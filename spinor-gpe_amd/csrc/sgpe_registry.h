@@ -116,8 +116,10 @@ void erow_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const InstDev* i, cons
   X(R, PR, 64, 2, 4) X(R, PR, 64, 4, 4) X(R, PR, 64, 4, 8) X(R, PR, 64, 8, 8) X(R, PR, 64, 8, 16)           \
   X(R, PR, 128, 2, 4) X(R, PR, 128, 4, 4) X(R, PR, 128, 4, 8) X(R, PR, 128, 8, 4) X(R, PR, 128, 8, 8)       \
   X(R, PR, 128, 8, 16) X(R, PR, 256, 2, 2) X(R, PR, 256, 2, 4) X(R, PR, 256, 4, 4) X(R, PR, 256, 4, 8)      \
-  X(R, PR, 256, 8, 4) X(R, PR, 256, 8, 8) X(R, PR, 256, 16, 8) X(R, PR, 512, 2, 2) X(R, PR, 512, 2, 4)      \
+  X(R, PR, 256, 8, 4) X(R, PR, 256, 8, 8) X(R, PR, 256, 8, 16) X(R, PR, 256, 16, 8) X(R, PR, 512, 2, 2)      \
+  X(R, PR, 512, 2, 4)                                                                                      \
   X(R, PR, 512, 4, 2) X(R, PR, 512, 4, 4) X(R, PR, 512, 8, 2) X(R, PR, 512, 8, 4) X(R, PR, 512, 8, 8)       \
+  X(R, PR, 512, 8, 16)                                                                                     \
   X(R, PR, 512, 16, 4) X(R, PR, 512, 16, 8) X(R, PR, 1024, 4, 2) X(R, PR, 1024, 4, 4) X(R, PR, 1024, 8, 2)  \
   X(R, PR, 1024, 8, 4) X(R, PR, 1024, 16, 4) X(R, PR, 1024, 16, 8) X(R, PR, 2048, 4, 2) X(R, PR, 2048, 8, 2) \
   X(R, PR, 2048, 8, 4) X(R, PR, 2048, 16, 2) X(R, PR, 2048, 16, 4) X(R, PR, 4096, 8, 2) X(R, PR, 4096, 16, 2) \
