@@ -1,0 +1,70 @@
+"""Host-side pieces of the benchmark: the roofline arithmetic and the
+consistency of the shipped MI355X tuning table with the measured-traffic
+summary that ``bench.py`` reads (every pass kernel the shipped table selects
+for the benchmark workloads has a rocprofv3 traffic entry)."""
+
+import json
+import os
+
+import pytest
+
+import bench
+from spinor_gpe.engine import autotune
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(autouse=True)
+def _packaged_table_only(monkeypatch, tmp_path):
+    """Look shapes up in the shipped table, not in a user cache."""
+    monkeypatch.setenv('SGPE_TUNE_FILE', str(tmp_path / 'none.json'))
+
+
+def _instance(kind, prec, n, shape):
+    real = 'double' if prec == 'c128' else 'float'
+    e, p, stream = (shape[0], shape[1], shape[4]) if kind == 'row' else (shape[2], shape[3], shape[5])
+    return f"{kind}_pass_kernel<{real}, {n}, {e}, {p}>" + (' streaming' if stream else '')
+
+
+@pytest.mark.parametrize('prec', ['c128', 'c64'])
+@pytest.mark.parametrize('n', [512, 2048])
+def test_shipped_shapes_have_traffic_entries(prec, n):
+    shape = autotune.lookup(prec, n, n, 1)
+    assert shape is not None and len(shape) >= 6, shape
+    for kind in ('row', 'col'):
+        name = _instance(kind, prec, n, shape)
+        traffic = bench.measured_traffic(name)
+        assert traffic is not None, f"no measured traffic for {name}"
+        alg = 2 * 2 * n * n * (16 if prec == 'c128' else 8)
+        assert 1.0 <= traffic / alg < 1.1, (name, traffic / alg)
+
+
+def test_roofline_arithmetic():
+    res = {'row_ms': 0.0125, 'col_ms': 0.010, 'tuning': (4, 2, 8, 4), 'streaming': (1, 1)}
+    r = bench.roofline(res, 'c128', 512)
+    assert r['kernel'] == 'row_pass'
+    assert r['kernel_instance'] == 'row_pass_kernel<double, 512, 4, 2> streaming'
+    assert r['algorithmic_bytes'] == 2 * 2 * 512 * 512 * 16
+    assert r['achieved'] == pytest.approx(16777216 / 12.5e-6 / 1e9, abs=0.05)  # rounded to 0.1 GB/s
+    assert r['frac'] == pytest.approx(r['achieved'] / 8000.0, abs=1e-4)
+    res = {'row_ms': 0.005, 'col_ms': 0.007, 'tuning': (8, 1, 16, 4), 'streaming': (1, 0)}
+    r = bench.roofline(res, 'c64', 2048)
+    assert r['kernel'] == 'col_pass'
+    assert r['kernel_instance'] == 'col_pass_kernel<float, 2048, 16, 4>'
+    assert r['traffic'] == bench.measured_traffic('col_pass_kernel<float, 2048, 16, 4>')
+
+
+def test_traffic_summary_is_consistent():
+    with open(os.path.join(ROOT, bench.TRAFFIC_FILE), encoding='utf8') as f:
+        data = json.load(f)
+    for name, k in data['kernels'].items():
+        prec = 'c64' if '<float' in name else 'c128'
+        cal = [v for key, v in data['calibration'].items() if key.startswith(prec)]
+        assert cal, name
+        # traffic = fetch x fetch factor + write x write factor, in bytes
+        ok = any(abs(k['traffic_bytes'] - 1024 * (c['fetch_factor'] * k['fetch_kib'] + c['write_factor']
+                                                  * k['write_kib'])) <= 0.001 * k['traffic_bytes']
+                 for c in cal)
+        assert ok, name
+        assert k['traffic_over_algorithmic'] == pytest.approx(k['traffic_bytes'] / k['algorithmic_bytes'],
+                                                              abs=1e-3)
