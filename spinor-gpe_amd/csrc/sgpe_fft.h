@@ -371,4 +371,41 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* scratch) {
   __syncthreads();
 }
 
+// Block-wide sum for the end of a kernel: the same fixed tree order as
+// block_sum, but only thread 0 gets the result. `scratch` (NV * waves
+// doubles) must be LDS that no other code of the kernel touches, so that
+// one barrier suffices (block_sum needs three: its scratch may overlap
+// buffers still being read, and it is reused afterwards).
+template <int NV>
+__device__ __forceinline__ void block_sum_thread0(double (&v)[NV], double* scratch) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nwave = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v[i] += __shfl_xor(v[i], off, 64);
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) scratch[i * nwave + wave] = v[i];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      double s = 0.0;
+      for (int w = 0; w < nwave; ++w) s += scratch[i * nwave + w];
+      v[i] = s;
+    }
+  }
+}
+
+// LDS of the column pass: its exchange buffer (W columns of n points), then
+// kColSumBytes for the end-of-pass sums (2 doubles per wave, up to 16
+// waves), then the twiddle and kinetic tables when they fit. Same on host
+// and device.
+constexpr size_t kColSumBytes = 256;
+constexpr size_t col_data_bytes(int w, int n, size_t cs) { return (size_t)w * n * cs + kColSumBytes; }
+
 }  // namespace sgpe
