@@ -180,6 +180,13 @@ static size_t lds_bytes(size_t data_bytes, int n, size_t cs, int tables = 0) {
 }
 static const int kColTables = 2;
 
+// Dynamic LDS of a column pass: exchange buffer, tables, and the separate
+// slot for its end-of-pass sums when it fits (col_sum_separate).
+static size_t col_lds_bytes(int w, int n, size_t cs) {
+  const size_t base = lds_bytes((size_t)w * n * cs, n, cs, kColTables);
+  return base + (col_sum_separate(base) ? kColSumBytes : 0);
+}
+
 // Dynamic LDS of a row pass: exchange buffers and twiddles, plus a second
 // exchange buffer when the kernel double-buffers (row_double_buffer).
 static size_t row_lds_bytes(int e, size_t data_bytes, int n, size_t cs) {
@@ -246,9 +253,9 @@ static int plan_select(sgpe_plan* p, int row_e, int row_rows, int col_e, int col
   if (p->ny % row_rows || p->nx % col_w) return fail(SGPE_E_UNSUPPORTED, "tile does not divide the grid");
   const size_t cs = csize(p->prec);
   const size_t row_data = (size_t)row_rows * 2 * p->nx * cs;
-  const size_t col_data = col_data_bytes(col_w, p->ny, cs);
+  const size_t col_data = (size_t)col_w * p->ny * cs;
   if (row_data > kMaxLds || col_data > kMaxLds) return fail(SGPE_E_UNSUPPORTED, "FFT shape exceeds LDS");
-  const size_t rl = row_lds_bytes(row_e, row_data, p->nx, cs), cl = lds_bytes(col_data, p->ny, cs, kColTables);
+  const size_t rl = row_lds_bytes(row_e, row_data, p->nx, cs), cl = col_lds_bytes(col_w, p->ny, cs);
   int rc = set_lds_attr(r->kernel, rl);
   if (!rc) rc = set_lds_attr(r->kernel_g, rl);
   if (!rc) rc = set_lds_attr(c->kernel, cl);
@@ -271,7 +278,7 @@ static int plan_select(sgpe_plan* p, int row_e, int row_rows, int col_e, int col
 
 static void launch_col(sgpe_plan* p, hipStream_t s, int mode, int dprev, int dcur, hipEvent_t e0 = nullptr,
                        hipEvent_t e1 = nullptr) {
-  const size_t lds = lds_bytes(col_data_bytes(p->col_w, p->ny, csize(p->prec)), p->ny, csize(p->prec), kColTables);
+  const size_t lds = col_lds_bytes(p->col_w, p->ny, csize(p->prec));
   dim3 grid(p->nx / p->col_w, 2, p->nslots), block(p->col_w * (p->ny / p->col_e));
   ((ColFn)(p->dense_kin ? p->col->launch_g : p->col->launch))(grid, block, lds, s, e0, e1, p->d_inst, p->d_run,
                                                               p->tw_y, mode, dprev, dcur, p->col_stream);

@@ -401,11 +401,11 @@ __device__ __forceinline__ void block_sum_thread0(double (&v)[NV], double* scrat
   }
 }
 
-// LDS of the column pass: its exchange buffer (W columns of n points), then
-// kColSumBytes for the end-of-pass sums (2 doubles per wave, up to 16
-// waves), then the twiddle and kinetic tables when they fit. Same on host
-// and device.
+// LDS for the column pass's end-of-pass sums (2 doubles per wave, up to 16
+// waves), placed after its exchange buffer and tables when that still fits
+// in LDS (same predicate on host and device); otherwise the sums reuse the
+// exchange buffer (block_sum).
 constexpr size_t kColSumBytes = 256;
-constexpr size_t col_data_bytes(int w, int n, size_t cs) { return (size_t)w * n * cs + kColSumBytes; }
+constexpr bool col_sum_separate(size_t base_bytes) { return base_bytes + kColSumBytes <= kLdsBytes; }
 
 }  // namespace sgpe

@@ -232,12 +232,13 @@ __global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __re
   typedef typename Cx<R>::T C;
   typedef LdsView<W, 1 << 30, 0> V;
   constexpr int T = N / E;
-  // LDS: [W*N exchange][sums][N twiddles][N ky_prev][N ky_cur] (tables
-  // when they fit)
-  constexpr size_t kData = col_data_bytes(W, N, sizeof(C));
-  constexpr int kTab = (int)(kData / sizeof(C));  // first table element
+  // LDS: [W*N exchange][N twiddles][N ky_prev][N ky_cur][sums] (tables and
+  // a separate sum slot when they fit)
+  constexpr size_t kData = (size_t)W * N * sizeof(C);
   constexpr bool kTw = tw_in_lds(kData, N, sizeof(C));
   constexpr bool kKy = kTw && kData + 3 * (size_t)N * sizeof(C) <= kLdsBytes;
+  constexpr size_t kBase = kData + (kTw ? (size_t)N * sizeof(C) : 0) + (kKy ? 2 * (size_t)N * sizeof(C) : 0);
+  constexpr bool kSumSep = col_sum_separate(kBase);
   extern __shared__ double smem[];
   const int inst = blockIdx.z;
   const InstDev& P = insts[inst];
@@ -281,9 +282,9 @@ __global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __re
   sp.load(td ? tws : reinterpret_cast<const C*>(P.ky[d_prev][comp]));
   sc_.load(td ? tws : reinterpret_cast<const C*>(P.ky[d_cur][comp]));
   stw.load(tws);
-  const C* kyp = sp.store(sm + kTab + N);
-  const C* kyc = sc_.store(sm + kTab + 2 * N);
-  const C* tw = stw.store(sm + kTab);
+  const C* kyp = sp.store(sm + W * N + N);
+  const C* kyc = sc_.store(sm + W * N + 2 * N);
+  const C* tw = stw.store(sm + W * N);
 
   if (!kspace)
     fft_run<N, E, false, V>(a, t, tw, lds);
@@ -312,7 +313,10 @@ __global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __re
   }
 
   double v[2] = {s_acc, n_acc};
-  block_sum_thread0<2>(v, reinterpret_cast<double*>(sm + W * N));
+  if constexpr (kSumSep)
+    block_sum_thread0<2>(v, reinterpret_cast<double*>(reinterpret_cast<char*>(smem) + kBase));
+  else
+    block_sum<2>(v, smem);
   if (threadIdx.x == 0) {
     const int nt = P.nx / W;
     P.part[PART_S * 2 * nt + comp * nt + tile] = v[0];
