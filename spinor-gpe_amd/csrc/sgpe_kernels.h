@@ -574,22 +574,20 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
   else
     fft_run2<N, E, true, V>(a0, a1, t, tw, lds0, lds1);
 
-  // The x-table values of this thread's points are loaded together, so the
-  // block waits for memory once per group rather than once per point: with
-  // the state at the start when E <= 2 (few registers to hold them through
-  // the inverse transform), otherwise after it, in two halves. An absent
-  // potential table is stood in for by the global twiddle table (N entries;
-  // unused).
-  // Radix 4: the tables of the second half of the points are loaded only
-  // after the first half is done (the scheduling barrier keeps the compiler
-  // from hoisting those loads), so at most half of them are live at once:
-  // 216 -> 162 VGPRs for the complex128 radix-4 pass (3 waves per SIMD
-  // instead of 2), paid for with one more L2 round trip per row. Complex128
-  // radix >= 8: point by point, each point's tables loaded while the
-  // previous point is computed (296 -> 224 VGPRs, 2 waves per SIMD; with the
-  // twiddles in global memory two 2048-point workgroups share a CU). In
-  // complex64 the halves already fit 3 waves per SIMD, and the point-by-
-  // point loads only add latency.
+  // The x-table values of this thread's points (an absent potential table
+  // is stood in for by the global twiddle table: N entries, unused). Radix
+  // 2: loaded with the state at the start (few registers to hold them
+  // through the inverse transform). Radix 4: after the inverse transform,
+  // in two halves, the second loaded only after the first half of the
+  // points is done (the scheduling barrier keeps the compiler from hoisting
+  // those loads), so at most half of them are live at once: 216 -> 162
+  // VGPRs for the complex128 pass (3 waves per SIMD instead of 2), paid for
+  // with one more L2 round trip per row. Complex128 radix >= 8: point by
+  // point, each point's tables loaded while the previous point is computed
+  // (296 -> 224 VGPRs, 2 waves per SIMD; with the twiddles in global memory
+  // two 2048-point workgroups share a CU). Complex64 radix >= 8 uses the
+  // halves, which already fit 3 waves per SIMD (point by point would only
+  // add latency).
   if constexpr (E >= 8 && sizeof(C) == 16) {
     load_x_tables<E, T, 0, 1>(tx0, tx1, tex, px0, px1, eth, tws, t);
     pointwise_points<R, N, E, T, !GENERAL, 0>(a0, a1, P, dur, y, t, lam2, py0, py1, tx0, tx1, tex, px0, px1, eth,
