@@ -95,8 +95,26 @@ def time_engine(ps, device, precision, steps, warmup, block, prof_steps, tune, d
     plan.sync()
     st = plan.states[0]
     atoms = float(torch.sum(torch.abs(st) ** 2).item()) * ps.space['dv_k']
+    energy = plan.energy()[0].cpu().numpy()
     return {'elapsed': elapsed, 'col_ms': col_ms / n_col, 'row_ms': row_ms / n_row,
-            'tuning': plan.tuning(), 'streaming': plan.streaming(), 'atoms': atoms, 'blocks': blocks}
+            'tuning': plan.tuning(), 'streaming': plan.streaming(), 'atoms': atoms, 'blocks': blocks,
+            'state': st.cpu().numpy().astype(np.complex128), 'energy': energy,
+            'steps_total': warmup + steps + BLOCK_REPS * block + prof_steps}
+
+
+def c64_vs_c128(r64, r128, dv_k):
+    """Errors of the complex64 run against the complex128 run of the same
+    steps from the same state: state fidelity |<a|b>|^2 / (<a|a><b|b>),
+    relative GP energy difference, and population differences (of N_at)."""
+    a, b = r64['state'], r128['state']
+    fid = abs(np.vdot(b, a)) ** 2 / (np.vdot(a, a).real * np.vdot(b, b).real)
+    pops_a = [float(np.sum(np.abs(c) ** 2)) * dv_k for c in a]
+    pops_b = [float(np.sum(np.abs(c) ** 2)) * dv_k for c in b]
+    n_at = sum(pops_b)
+    e64, e128 = float(r64['energy'][0]), float(r128['energy'][0])
+    return {'steps': r64['steps_total'], 'state_fidelity': float(fid),
+            'energy_rel_err': abs(e64 - e128) / abs(e128),
+            'pops_err_of_atoms': max(abs(x - y) for x, y in zip(pops_a, pops_b)) / n_at}
 
 
 def sweep_leg(device, check_every, fp64_sample):
@@ -282,16 +300,19 @@ def large_leg(args, device):
     n = 2048
     out = {'workload': '2048x2048 real-time Raman-coupled GPE (kL=3, Omega=1.5 E_R, lab frame), '
                        '64 vortices on component 0, 1e6 atoms, dt=5e-4, 1 instance'}
+    runs = {}
     for prec in args.precisions:
         log(f"timing 2048^2 {prec}: warmup {args.large_warmup}, steps {args.large_steps}")
-        r = time_engine(ps, device, prec, args.large_steps, args.large_warmup, args.block,
-                        args.profile_steps // 4, False, dt=5e-4)
+        r = runs[prec] = time_engine(ps, device, prec, args.large_steps, args.large_warmup, args.block,
+                                     args.profile_steps // 4, False, dt=5e-4)
         sps = args.large_steps / r['elapsed']
         out[prec] = {'value': round(sps, 2), 'unit': 'split-steps/s',
                      'ms_per_step': round(r['elapsed'] / args.large_steps * 1e3, 5),
                      'grid_point_steps_per_s': round(sps * n * n, 1), 'block_timing': r['blocks'],
                      'tuning_row_e_rows_col_e_w': list(r['tuning']), 'streaming_row_col': list(r['streaming']),
                      'roofline': roofline(r, prec, n), 'atoms_after_run': r['atoms']}
+    if 'c64' in runs and 'c128' in runs:
+        out['c64']['vs_c128'] = c64_vs_c128(runs['c64'], runs['c128'], ps.space['dv_k'])
     return out
 
 
@@ -332,6 +353,7 @@ def step_leg(args, device):
         line['c64'] = {'value': round(sps64, 2), 'ms_per_step': round(r64['elapsed'] / args.steps * 1e3, 5),
                        'grid_point_steps_per_s': round(sps64 * n_pts, 1), 'block_timing': r64['blocks'],
                        'roofline': roofline(r64, 'c64', 512),
+                       'vs_c128': c64_vs_c128(r64, r, ps.space['dv_k']),
                        'dtype': 'c64 (f32 arithmetic, f64 reductions)'}
     if not args.no_cpu_baseline:
         log(f"CPU baseline: oracle full_step() for ~{args.cpu_budget:.0f} s")

@@ -68,3 +68,22 @@ def test_traffic_summary_is_consistent():
         assert ok, name
         assert k['traffic_over_algorithmic'] == pytest.approx(k['traffic_bytes'] / k['algorithmic_bytes'],
                                                               abs=1e-3)
+
+
+def test_c64_vs_c128_errors():
+    import numpy as np  # pylint: disable=import-outside-toplevel
+    rng = np.random.default_rng(0)
+    b = rng.normal(size=(2, 8, 8)) + 1j * rng.normal(size=(2, 8, 8))
+    r128 = {'state': b, 'energy': np.array([2.0]), 'steps_total': 10}
+    same = bench.c64_vs_c128({'state': b * np.exp(0.3j), 'energy': np.array([2.0]), 'steps_total': 10},
+                             r128, 0.5)
+    assert same['state_fidelity'] == pytest.approx(1.0, abs=1e-14)  # a global phase is no error
+    assert same['energy_rel_err'] == 0.0 and same['pops_err_of_atoms'] == pytest.approx(0.0, abs=1e-15)
+    swapped = np.stack([b[0], np.zeros_like(b[1])])
+    other = {'state': np.stack([np.zeros_like(b[0]), b[1]]), 'energy': np.array([2.2]), 'steps_total': 10}
+    res = bench.c64_vs_c128(other, {'state': swapped, 'energy': np.array([2.0]), 'steps_total': 10}, 1.0)
+    assert res['state_fidelity'] == pytest.approx(0.0, abs=1e-15)  # orthogonal states
+    assert res['energy_rel_err'] == pytest.approx(0.1)
+    n0 = float(np.sum(np.abs(b[0]) ** 2))
+    n1 = float(np.sum(np.abs(b[1]) ** 2))
+    assert res['pops_err_of_atoms'] == pytest.approx(max(n0, n1) / n0)
