@@ -316,6 +316,72 @@ def large_leg(args, device):
     return out
 
 
+# The reference's propagation benchmark (spinor_gpe/benchmarks/benchmark_prop.py:
+# 28-40 grids; bench_data/TitanV_cuda_step.npz medians, ms per full_step).
+SIZES = [(64, 64), (64, 128), (128, 128), (128, 256), (256, 256), (256, 512), (512, 512),
+         (512, 1024), (1024, 1024), (1024, 2048), (2048, 2048), (2048, 4096), (4096, 4096)]
+TITAN_V_MS = {12: 3.774, 13: 3.785, 14: 3.842, 15: 3.951, 16: 4.062, 17: 5.285, 18: 6.708,
+              19: 12.375, 20: 22.370, 21: 41.160, 22: 79.821, 23: 158.277}
+
+
+def sizes_leg(args, device):
+    """The reference's own benchmark workload (benchmark_prop.py:48-67: 1e2
+    atoms, 50/50/2000 Hz trap, g = (1, 1, 1.04), +/-8 oscillator lengths,
+    Raman coupling set up with Omega = 0, imaginary time dt = 1/50,
+    complex128) on its 13 grids from 64x64 to 4096x4096. Per grid: the
+    reference's method (prop.full_step() timed call by call, here with a
+    device synchronize after each call; median and MAD) and the engine's
+    captured-graph rate (blocks of full steps timed with events; median and
+    MAD of the per-step time)."""
+    from spinor_gpe.pspinor import pspinor as spin
+    w = 2 * np.pi * 50
+    out = []
+    for grid in SIZES:
+        n_pts = grid[0] * grid[1]
+        log(f"sizes: {grid[0]}x{grid[1]}")
+        ps = spin.PSpinor(f'bp{grid[0]}x{grid[1]}', overwrite=True, atom_num=1e2,
+                          omeg={'x': w, 'y': w, 'z': 40 * w}, g_sc={'uu': 1, 'dd': 1, 'ud': 1.04},
+                          pop_frac=(0.5, 0.5), r_sizes=(8, 8), mesh_points=grid)
+        ps.coupling_setup(wavel=790.1e-9, kin_shift=False)
+        _, prop = ps.imaginary(1 / 50, 1, device, is_sampling=False, progress=False)
+        calls = []
+        for _ in range(args.sizes_calls):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            prop.full_step()
+            torch.cuda.synchronize()
+            calls.append(time.perf_counter() - t0)
+        calls = np.array(calls[3:]) * 1e3
+        plan = prop._plan  # pylint: disable=protected-access
+        block = max(1, min(args.block, int(2e6 // n_pts)))
+        plan.run(block, block=block)
+        events = [torch.cuda.Event(enable_timing=True) for _ in range(args.sizes_blocks + 1)]
+        events[0].record()
+        for i in range(args.sizes_blocks):
+            plan.run(block, block=block)
+            events[i + 1].record()
+        torch.cuda.synchronize()
+        per = np.array([events[i].elapsed_time(events[i + 1]) / block for i in range(args.sizes_blocks)])
+        size = int(round(np.log2(n_pts)))
+        med_c, med_g = float(np.median(calls)), float(np.median(per))
+        row = {'grid': list(grid), 'log2_points': size,
+               'full_step_call_ms': {'median': round(med_c, 5),
+                                     'mad': round(float(np.median(np.abs(calls - med_c))), 6)},
+               'graph_ms_per_step': {'median': round(med_g, 5),
+                                     'mad': round(float(np.median(np.abs(per - med_g))), 6),
+                                     'steps_per_block': block},
+               'split_steps_per_s': round(1e3 / med_g, 1)}
+        if size in TITAN_V_MS:
+            row['titan_v_ms'] = TITAN_V_MS[size]
+            row['speedup_vs_titan_v'] = {'full_step_call': round(TITAN_V_MS[size] / med_c, 2),
+                                         'graph': round(TITAN_V_MS[size] / med_g, 2)}
+        out.append(row)
+        del prop, plan
+        torch.cuda.empty_cache()
+    return {'workload': 'reference benchmark_prop.py: 1e2 atoms, r=8, g_ud=1.04, Raman set up with '
+                        'Omega=0, imaginary dt=1/50, complex128', 'grids': out}
+
+
 def step_leg(args, device):
     """The metric value: 512^2 split-steps/s in complex128 (and complex64)."""
     log("building the 512^2 initial state (256^2 ground state, 4000 imaginary steps)")
@@ -374,11 +440,14 @@ def main():
                     help='re-measure the FFT shapes of the workload instead of using the shipped table')
     ap.add_argument('--precisions', nargs='+', default=['c128', 'c64'], choices=['c128', 'c64'])
     ap.add_argument('--legs', nargs='+', default=['step', 'large', 'sweep', 'serve'],
-                    choices=['step', 'large', 'sweep', 'serve'],
+                    choices=['step', 'large', 'sweep', 'serve', 'sizes'],
                     help='step: 512^2 split-step rate (the metric value); large: the 2048^2 '
                          'workload; sweep: batched ground states; serve: continuous-batching '
-                         'request stream')
+                         'request stream; sizes (not run by default): the reference\'s own '
+                         'benchmark workload on its 13 grids, beside its Titan V times')
     ap.add_argument('--large-steps', type=int, default=1000)
+    ap.add_argument('--sizes-calls', type=int, default=53, help='timed full_step() calls per grid (sizes leg)')
+    ap.add_argument('--sizes-blocks', type=int, default=10, help='timed graph blocks per grid (sizes leg)')
     ap.add_argument('--large-warmup', type=int, default=100)
     ap.add_argument('--requests', type=int, default=48, help='requests of the served stream')
     ap.add_argument('--load', type=float, default=0.8, help='arrival rate / measured capacity')
@@ -410,6 +479,8 @@ def main():
     line = step_leg(args, device) if 'step' in args.legs else {'metric': METRIC, 'value': None}
     if 'large' in args.legs:
         line['large'] = large_leg(args, device)
+    if 'sizes' in args.legs:
+        line['sizes'] = sizes_leg(args, device)
     if 'sweep' in args.legs:
         line['sweep'] = sweep_leg(device, args.sweep_check, args.fp64_sample)
     if 'serve' in args.legs:
