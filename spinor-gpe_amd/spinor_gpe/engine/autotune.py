@@ -58,15 +58,32 @@ def _load(path):
         return {}
 
 
-def lookup(precision, ny, nx, n_slots):
-    """Cached (row_e, row_rows, col_e, col_w[, row_stream, col_stream]) or
-    None (user cache, then packaged)."""
+def _lookup_exact(precision, ny, nx, n_slots):
     k = key(precision, ny, nx, n_slots)
     for path in (cache_path(), PACKAGED):
         entry = _load(path).get(k)
         if entry:
             return tuple(entry['shape'])
     return None
+
+
+def lookup(precision, ny, nx, n_slots):
+    """Cached (row_e, row_rows, col_e, col_w[, row_stream, col_stream]) or
+    None (user cache, then packaged). A grid without an entry of its own
+    takes its row-pass choice from the nx x nx entry and its column-pass
+    choice from the ny x ny entry (the row transforms have length nx, the
+    column transforms length ny)."""
+    exact = _lookup_exact(precision, ny, nx, n_slots)
+    if exact or ny == nx:
+        return exact
+    rows = _lookup_exact(precision, nx, nx, n_slots)
+    cols = _lookup_exact(precision, ny, ny, n_slots)
+    if not rows or not cols:
+        return None
+    shape = [rows[0], rows[1], cols[2], cols[3]]
+    if len(rows) >= 6 and len(cols) >= 6:
+        shape += [rows[4], cols[5]]
+    return tuple(shape)
 
 
 def store(precision, ny, nx, n_slots, shape, timings):

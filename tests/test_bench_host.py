@@ -87,3 +87,11 @@ def test_c64_vs_c128_errors():
     n0 = float(np.sum(np.abs(b[0]) ** 2))
     n1 = float(np.sum(np.abs(b[1]) ** 2))
     assert res['pops_err_of_atoms'] == pytest.approx(max(n0, n1) / n0)
+
+
+def test_non_square_grids_combine_square_tunings():
+    sq_rows = autotune.lookup('c128', 1024, 1024, 1)
+    sq_cols = autotune.lookup('c128', 512, 512, 1)
+    got = autotune.lookup('c128', 512, 1024, 1)  # ny = 512, nx = 1024
+    assert got[:2] == sq_rows[:2] and got[2:4] == sq_cols[2:4]
+    assert got[4] == sq_rows[4] and got[5] == sq_cols[5]
