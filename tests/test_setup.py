@@ -143,3 +143,25 @@ def test_shift_momentum_moves_peak():
     assert [np.sum(np.abs(p) ** 2) for p in ps.psik] == pytest.approx(norms, rel=1e-13)
     np.testing.assert_allclose(ps.psi[0], np.fft.ifftn(np.fft.ifftshift(ps.psik[0]))
                                / (ps.space['dv_r'] / (2 * np.pi)), atol=1e-12)
+
+
+@pytest.mark.parametrize('s', [3, 4])
+def test_shift_momentum_integer_shift_closed_form(s):
+    """For a whole-pixel shift s the Fourier shift of pspinor.py:534-545 is
+    exact: with the transform pair of tensor_tools.py:201-258 around it, each
+    component's k-array becomes (-1)^s (f_a roll(psik, +s) + f_b roll(psik, -s))
+    along x, where (f_a, f_b) = frac for component 0 and its flip for
+    component 1 (the sign comes from applying fftfreq phases to an
+    fftshifted array). A random state, so that mirrored or unflipped shifts
+    cannot pass."""
+    ps = c0(mesh_points=(64, 128), r_sizes=(8, 16))
+    ps.coupling_setup(scale=0.4)
+    rng = np.random.default_rng(5)
+    p = [rng.normal(size=(64, 128)) + 1j * rng.normal(size=(64, 128)) for _ in range(2)]
+    frac = (0.3, 0.7)
+    ps.shift_momentum(psik=[q.copy() for q in p], scale=s * ps.space['dk'][0] / ps.kL_recoil, frac=frac)
+    sign = (-1) ** s
+    want = [sign * (frac[0] * np.roll(p[0], s, axis=1) + frac[1] * np.roll(p[0], -s, axis=1)),
+            sign * (frac[1] * np.roll(p[1], s, axis=1) + frac[0] * np.roll(p[1], -s, axis=1))]
+    for got, w in zip(ps.psik, want):
+        np.testing.assert_allclose(got, w, rtol=0, atol=1e-13 * np.max(np.abs(w)))
