@@ -165,3 +165,31 @@ def test_shift_momentum_integer_shift_closed_form(s):
             sign * (frac[1] * np.roll(p[1], s, axis=1) + frac[0] * np.roll(p[1], -s, axis=1))]
     for got, w in zip(ps.psik, want):
         np.testing.assert_allclose(got, w, rtol=0, atol=1e-13 * np.max(np.abs(w)))
+
+
+@pytest.mark.parametrize('w', [1, -1])
+def test_seed_vortices_profile_and_orientation(w):
+    """The imprint of pspinor.py:731-743 on a grid point (x0, y0): k pixels
+    along +x the state is scaled by rho/sqrt(rho^2 + 1), rho = k dx / xi, with
+    no phase; k pixels along +y by the same modulus and the phase w*pi/2;
+    along -x by the phase pi. This fixes the winding sign, the atan2(dy, dx)
+    orientation and the healing-length scaling that a closed loop's
+    circulation alone would not."""
+    ps = c0(mesh_points=(128, 128), r_sizes=(8, 8))
+    x, y = ps.space['x'], ps.space['y']
+    ix0, iy0 = 70, 60
+    before = [p.copy() for p in ps.psi]
+    ps.seed_vortices([(x[ix0], y[iy0])], [w])
+    dx, dy = ps.space['dr']
+    for i in range(2):
+        with np.errstate(divide='ignore', invalid='ignore'):  # zero outside the Thomas-Fermi radius
+            ratio = ps.psi[i] / before[i]
+        assert ratio[iy0, ix0] == 0
+        for k in (1, 3, 8):
+            for (jy, jx), dist, phase in (((iy0, ix0 + k), k * dx, 0.0), ((iy0 + k, ix0), k * dy, w * np.pi / 2),
+                                          ((iy0, ix0 - k), k * dx, np.pi)):
+                rho = dist / ps.heal[i]
+                assert abs(ratio[jy, jx]) == pytest.approx(rho / np.sqrt(rho ** 2 + 1), rel=1e-12)
+                assert np.angle(ratio[jy, jx] * np.exp(-1j * phase)) == pytest.approx(0.0, abs=1e-12)
+    c = ps.space['dv_r'] / (2 * np.pi)
+    np.testing.assert_allclose(ps.psik[1], np.fft.fftshift(np.fft.fftn(ps.psi[1])) * c, atol=1e-12)
