@@ -34,7 +34,7 @@ enum {
   SGPE_OK = 0,
   SGPE_E_ARG = -1,         /* invalid argument */
   SGPE_E_HIP = -2,         /* HIP runtime error */
-  SGPE_E_NOMEM = -3,       /* device allocation failed */
+  SGPE_E_NOMEM = -3,       /* device or host allocation failed */
   SGPE_E_UNSUPPORTED = -4  /* grid size / precision not supported */
 };
 

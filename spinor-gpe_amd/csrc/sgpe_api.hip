@@ -6,7 +6,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <map>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -20,8 +22,26 @@ using namespace sgpe;
 static thread_local std::string g_err;
 
 static int fail(int code, const std::string& msg) {
-  g_err = msg;
+  try {
+    g_err = msg;
+  } catch (...) {
+  }
   return code;
+}
+
+// Runs an entry point's body so that no C++ exception crosses the C ABI: a
+// failed host allocation (std::bad_alloc from containers or new) becomes
+// SGPE_E_NOMEM, anything else SGPE_E_HIP, each with a message.
+template <class F> static int guarded(F&& body) noexcept {
+  try {
+    return body();
+  } catch (const std::bad_alloc&) {
+    return fail(SGPE_E_NOMEM, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(SGPE_E_HIP, std::string("internal error: ") + e.what());
+  } catch (...) {
+    return fail(SGPE_E_HIP, "internal error");
+  }
 }
 
 #define HIP_TRY(expr)                                                                             \
@@ -615,38 +635,44 @@ int sgpe_abi_version(void) { return SGPE_ABI_VERSION; }
 const char* sgpe_last_error(void) { return g_err.c_str(); }
 
 int sgpe_ctx_create(int device, void* stream, sgpe_ctx** out) {
-  if (!out) return fail(SGPE_E_ARG, "out is NULL");
-  *out = nullptr;
-  int n = 0;
-  HIP_TRY(hipGetDeviceCount(&n));
-  if (device < 0 || device >= n) return fail(SGPE_E_ARG, "no such device");
-  HIP_TRY(hipSetDevice(device));
-  sgpe_ctx* c = new sgpe_ctx();
-  c->device = device;
-  c->stream = (hipStream_t)stream;
-  if (hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking) != hipSuccess) {
-    delete c;
-    return fail(SGPE_E_HIP, "capture stream creation failed");
-  }
-  if (hipHostMalloc((void**)&c->red_host, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
-    (void)hipStreamDestroy(c->cap_stream);
-    delete c;
-    return fail(SGPE_E_HIP, "pinned buffer allocation failed");
-  }
-  *out = c;
-  return SGPE_OK;
+  return guarded([&]() -> int {
+    if (!out) return fail(SGPE_E_ARG, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(SGPE_E_ARG, "no such device");
+    HIP_TRY(hipSetDevice(device));
+    sgpe_ctx* c = new sgpe_ctx();
+    c->device = device;
+    c->stream = (hipStream_t)stream;
+    if (hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking) != hipSuccess) {
+      delete c;
+      return fail(SGPE_E_HIP, "capture stream creation failed");
+    }
+    if (hipHostMalloc((void**)&c->red_host, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+      (void)hipStreamDestroy(c->cap_stream);
+      delete c;
+      return fail(SGPE_E_HIP, "pinned buffer allocation failed");
+    }
+    *out = c;
+    return SGPE_OK;
+  });
 }
 
 int sgpe_ctx_set_stream(sgpe_ctx* ctx, void* stream) {
-  if (!ctx) return fail(SGPE_E_ARG, "ctx is NULL");
-  ctx->stream = (hipStream_t)stream;
-  return SGPE_OK;
+  return guarded([&]() -> int {
+    if (!ctx) return fail(SGPE_E_ARG, "ctx is NULL");
+    ctx->stream = (hipStream_t)stream;
+    return SGPE_OK;
+  });
 }
 
 int sgpe_ctx_synchronize(sgpe_ctx* ctx) {
-  if (!ctx) return fail(SGPE_E_ARG, "ctx is NULL");
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
-  return SGPE_OK;
+  return guarded([&]() -> int {
+    if (!ctx) return fail(SGPE_E_ARG, "ctx is NULL");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SGPE_OK;
+  });
 }
 
 void sgpe_ctx_destroy(sgpe_ctx* ctx) {
@@ -662,136 +688,144 @@ void sgpe_ctx_destroy(sgpe_ctx* ctx) {
 
 int sgpe_fft2(sgpe_ctx* ctx, int precision, int ny, int nx, int n_arrays, void* const* in, void* const* out,
               int inverse, double dx, double dy) {
-  if (!ctx || !in || !out || n_arrays < 0) return fail(SGPE_E_ARG, "bad arguments");
-  if (precision != SGPE_C128 && precision != SGPE_C64) return fail(SGPE_E_ARG, "bad precision");
-  if (!pow2_in_range(nx) || !pow2_in_range(ny))
-    return fail(SGPE_E_UNSUPPORTED, "grid sizes must be powers of two in [64, 4096]");
-  HIP_TRY(hipSetDevice(ctx->device));
-  int re, rr, ce, cw;
-  default_row(precision, nx, &re, &rr);
-  default_col(precision, ny, &ce, &cw);
-  const KEntry* fr = find_entry(K_FROW, precision, nx, re, rr);
-  const KEntry* fc = find_entry(K_FCOL, precision, ny, ce, cw);
-  if (!fr || !fc) return fail(SGPE_E_UNSUPPORTED, "FFT shape not compiled");
-  const size_t rl = lds_bytes((size_t)rr * nx * csize(precision), nx, csize(precision));
-  const size_t cl = lds_bytes((size_t)cw * ny * csize(precision), ny, csize(precision));
-  int rc = set_lds_attr(fr->kernel, rl);
-  if (rc) return rc;
-  if ((rc = set_lds_attr(fc->kernel, cl))) return rc;
-  const void *twx, *twy;
-  if ((rc = ctx_twiddles(ctx, precision, nx, re, &twx))) return rc;
-  if ((rc = ctx_twiddles(ctx, precision, ny, ce, &twy))) return rc;
-  const double c = dx * dy / (2.0 * M_PI);
-  dim3 rg(ny / rr), rb(rr * (nx / re)), cg(nx / cw), cb(cw * (ny / ce));
-  for (int i = 0; i < n_arrays; ++i) {
-    if (!in[i] || !out[i]) return fail(SGPE_E_ARG, "NULL array");
-    if (!inverse) {
-      ((FRowFn)fr->launch)(rg, rb, rl, ctx->stream, in[i], out[i], 0, 1, c, twx);
-      ((FColFn)fc->launch)(cg, cb, cl, ctx->stream, out[i], out[i], nx, 0, twy);
-    } else {
-      ((FColFn)fc->launch)(cg, cb, cl, ctx->stream, in[i], out[i], nx, 1, twy);
-      ((FRowFn)fr->launch)(rg, rb, rl, ctx->stream, out[i], out[i], 1, 1, 1.0 / ((double)nx * ny * c), twx);
+  return guarded([&]() -> int {
+    if (!ctx || !in || !out || n_arrays < 0) return fail(SGPE_E_ARG, "bad arguments");
+    if (precision != SGPE_C128 && precision != SGPE_C64) return fail(SGPE_E_ARG, "bad precision");
+    if (!pow2_in_range(nx) || !pow2_in_range(ny))
+      return fail(SGPE_E_UNSUPPORTED, "grid sizes must be powers of two in [64, 4096]");
+    HIP_TRY(hipSetDevice(ctx->device));
+    int re, rr, ce, cw;
+    default_row(precision, nx, &re, &rr);
+    default_col(precision, ny, &ce, &cw);
+    const KEntry* fr = find_entry(K_FROW, precision, nx, re, rr);
+    const KEntry* fc = find_entry(K_FCOL, precision, ny, ce, cw);
+    if (!fr || !fc) return fail(SGPE_E_UNSUPPORTED, "FFT shape not compiled");
+    const size_t rl = lds_bytes((size_t)rr * nx * csize(precision), nx, csize(precision));
+    const size_t cl = lds_bytes((size_t)cw * ny * csize(precision), ny, csize(precision));
+    int rc = set_lds_attr(fr->kernel, rl);
+    if (rc) return rc;
+    if ((rc = set_lds_attr(fc->kernel, cl))) return rc;
+    const void *twx, *twy;
+    if ((rc = ctx_twiddles(ctx, precision, nx, re, &twx))) return rc;
+    if ((rc = ctx_twiddles(ctx, precision, ny, ce, &twy))) return rc;
+    const double c = dx * dy / (2.0 * M_PI);
+    dim3 rg(ny / rr), rb(rr * (nx / re)), cg(nx / cw), cb(cw * (ny / ce));
+    for (int i = 0; i < n_arrays; ++i) {
+      if (!in[i] || !out[i]) return fail(SGPE_E_ARG, "NULL array");
+      if (!inverse) {
+        ((FRowFn)fr->launch)(rg, rb, rl, ctx->stream, in[i], out[i], 0, 1, c, twx);
+        ((FColFn)fc->launch)(cg, cb, cl, ctx->stream, out[i], out[i], nx, 0, twy);
+      } else {
+        ((FColFn)fc->launch)(cg, cb, cl, ctx->stream, in[i], out[i], nx, 1, twy);
+        ((FRowFn)fr->launch)(rg, rb, rl, ctx->stream, out[i], out[i], 1, 1, 1.0 / ((double)nx * ny * c), twx);
+      }
     }
-  }
-  HIP_TRY(hipGetLastError());
-  return SGPE_OK;
+    HIP_TRY(hipGetLastError());
+    return SGPE_OK;
+  });
 }
 
 static int reduce_blocks(size_t n) { return (int)std::min<size_t>((n + 255) / 256, 1024); }
 
 int sgpe_norm(sgpe_ctx* ctx, int precision, int ny, int nx, void* psi0, void* psi1, double vol_elem,
               double atom_num, void* dens0, void* dens1, double* nf_out) {
-  if (!ctx || !psi0 || !psi1 || ny <= 0 || nx <= 0) return fail(SGPE_E_ARG, "bad arguments");
-  if (precision != SGPE_C128 && precision != SGPE_C64) return fail(SGPE_E_ARG, "bad precision");
-  HIP_TRY(hipSetDevice(ctx->device));
-  const size_t n = (size_t)ny * nx;
-  const int nb = reduce_blocks(n);
-  int rc = ctx_reduction(ctx, 2 * (size_t)nb + 1);
-  if (rc) return rc;
-  double* nf_dev = ctx->red + 2 * nb;
-  if (precision == SGPE_C128) {
-    hipLaunchKernelGGL(density_sums_kernel<double>, dim3(nb), dim3(256), 0, ctx->stream, (const double2*)psi0,
-                       (const double2*)psi1, n, ctx->red);
-    hipLaunchKernelGGL(norm_apply_kernel<double>, dim3(nb), dim3(256), 0, ctx->stream, (double2*)psi0,
-                       (double2*)psi1, n, (const double*)ctx->red, nb, vol_elem, atom_num, (double*)dens0,
-                       (double*)dens1, nf_dev);
-  } else {
-    hipLaunchKernelGGL(density_sums_kernel<float>, dim3(nb), dim3(256), 0, ctx->stream, (const float2*)psi0,
-                       (const float2*)psi1, n, ctx->red);
-    hipLaunchKernelGGL(norm_apply_kernel<float>, dim3(nb), dim3(256), 0, ctx->stream, (float2*)psi0,
-                       (float2*)psi1, n, (const double*)ctx->red, nb, vol_elem, atom_num, (float*)dens0,
-                       (float*)dens1, nf_dev);
-  }
-  HIP_TRY(hipGetLastError());
-  if (nf_out) {
-    HIP_TRY(hipMemcpyAsync(ctx->red_host, nf_dev, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    *nf_out = ctx->red_host[0];
-  }
-  return SGPE_OK;
+  return guarded([&]() -> int {
+    if (!ctx || !psi0 || !psi1 || ny <= 0 || nx <= 0) return fail(SGPE_E_ARG, "bad arguments");
+    if (precision != SGPE_C128 && precision != SGPE_C64) return fail(SGPE_E_ARG, "bad precision");
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t n = (size_t)ny * nx;
+    const int nb = reduce_blocks(n);
+    int rc = ctx_reduction(ctx, 2 * (size_t)nb + 1);
+    if (rc) return rc;
+    double* nf_dev = ctx->red + 2 * nb;
+    if (precision == SGPE_C128) {
+      hipLaunchKernelGGL(density_sums_kernel<double>, dim3(nb), dim3(256), 0, ctx->stream, (const double2*)psi0,
+                         (const double2*)psi1, n, ctx->red);
+      hipLaunchKernelGGL(norm_apply_kernel<double>, dim3(nb), dim3(256), 0, ctx->stream, (double2*)psi0,
+                         (double2*)psi1, n, (const double*)ctx->red, nb, vol_elem, atom_num, (double*)dens0,
+                         (double*)dens1, nf_dev);
+    } else {
+      hipLaunchKernelGGL(density_sums_kernel<float>, dim3(nb), dim3(256), 0, ctx->stream, (const float2*)psi0,
+                         (const float2*)psi1, n, ctx->red);
+      hipLaunchKernelGGL(norm_apply_kernel<float>, dim3(nb), dim3(256), 0, ctx->stream, (float2*)psi0,
+                         (float2*)psi1, n, (const double*)ctx->red, nb, vol_elem, atom_num, (float*)dens0,
+                         (float*)dens1, nf_dev);
+    }
+    HIP_TRY(hipGetLastError());
+    if (nf_out) {
+      HIP_TRY(hipMemcpyAsync(ctx->red_host, nf_dev, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+      HIP_TRY(hipStreamSynchronize(ctx->stream));
+      *nf_out = ctx->red_host[0];
+    }
+    return SGPE_OK;
+  });
 }
 
 int sgpe_pops(sgpe_ctx* ctx, int precision, int ny, int nx, const void* psi0, const void* psi1, double vol_elem,
               double* out2) {
-  if (!ctx || !psi0 || !psi1 || !out2 || ny <= 0 || nx <= 0) return fail(SGPE_E_ARG, "bad arguments");
-  if (precision != SGPE_C128 && precision != SGPE_C64) return fail(SGPE_E_ARG, "bad precision");
-  HIP_TRY(hipSetDevice(ctx->device));
-  const size_t n = (size_t)ny * nx;
-  const int nb = reduce_blocks(n);
-  int rc = ctx_reduction(ctx, 2 * (size_t)nb + 2);
-  if (rc) return rc;
-  double* res = ctx->red + 2 * nb;
-  if (precision == SGPE_C128)
-    hipLaunchKernelGGL(density_sums_kernel<double>, dim3(nb), dim3(256), 0, ctx->stream, (const double2*)psi0,
-                       (const double2*)psi1, n, ctx->red);
-  else
-    hipLaunchKernelGGL(density_sums_kernel<float>, dim3(nb), dim3(256), 0, ctx->stream, (const float2*)psi0,
-                       (const float2*)psi1, n, ctx->red);
-  hipLaunchKernelGGL(pops_final_kernel, dim3(1), dim3(256), 0, ctx->stream, (const double*)ctx->red, nb, vol_elem,
-                     res);
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpyAsync(ctx->red_host, res, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
-  out2[0] = ctx->red_host[0];
-  out2[1] = ctx->red_host[1];
-  return SGPE_OK;
+  return guarded([&]() -> int {
+    if (!ctx || !psi0 || !psi1 || !out2 || ny <= 0 || nx <= 0) return fail(SGPE_E_ARG, "bad arguments");
+    if (precision != SGPE_C128 && precision != SGPE_C64) return fail(SGPE_E_ARG, "bad precision");
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t n = (size_t)ny * nx;
+    const int nb = reduce_blocks(n);
+    int rc = ctx_reduction(ctx, 2 * (size_t)nb + 2);
+    if (rc) return rc;
+    double* res = ctx->red + 2 * nb;
+    if (precision == SGPE_C128)
+      hipLaunchKernelGGL(density_sums_kernel<double>, dim3(nb), dim3(256), 0, ctx->stream, (const double2*)psi0,
+                         (const double2*)psi1, n, ctx->red);
+    else
+      hipLaunchKernelGGL(density_sums_kernel<float>, dim3(nb), dim3(256), 0, ctx->stream, (const float2*)psi0,
+                         (const float2*)psi1, n, ctx->red);
+    hipLaunchKernelGGL(pops_final_kernel, dim3(1), dim3(256), 0, ctx->stream, (const double*)ctx->red, nb, vol_elem,
+                       res);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(ctx->red_host, res, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    out2[0] = ctx->red_host[0];
+    out2[1] = ctx->red_host[1];
+    return SGPE_OK;
+  });
 }
 
 int sgpe_plan_create(sgpe_ctx* ctx, int precision, int ny, int nx, int n_slots, sgpe_plan** out) {
-  if (!ctx || !out || n_slots <= 0) return fail(SGPE_E_ARG, "bad arguments");
-  *out = nullptr;
-  if (precision != SGPE_C128 && precision != SGPE_C64) return fail(SGPE_E_ARG, "bad precision");
-  if (!pow2_in_range(nx) || !pow2_in_range(ny))
-    return fail(SGPE_E_UNSUPPORTED, "grid sizes must be powers of two in [64, 4096]");
-  if (n_slots > 65535) return fail(SGPE_E_ARG, "too many slots");
-  HIP_TRY(hipSetDevice(ctx->device));
-  sgpe_plan* p = new sgpe_plan();
-  p->ctx = ctx;
-  p->prec = precision;
-  p->ny = ny;
-  p->nx = nx;
-  p->nslots = n_slots;
-  p->row_stream = p->col_stream = n_slots == 1 ? 1 : 0;
-  p->h_inst.resize(n_slots);
-  p->slots.resize(n_slots);
-  int re, rr, ce, cw;
-  default_row(precision, nx, &re, &rr);
-  default_col(precision, ny, &ce, &cw);
-  int rc = plan_select(p, re, rr, ce, cw);
-  if (!rc && hipMalloc((void**)&p->d_inst, n_slots * sizeof(InstDev)) != hipSuccess)
-    rc = fail(SGPE_E_NOMEM, "slot table allocation failed");
-  if (!rc && hipMalloc((void**)&p->d_run, n_slots * sizeof(RunDev)) != hipSuccess)
-    rc = fail(SGPE_E_NOMEM, "run state allocation failed");
-  if (!rc && hipMemset(p->d_run, 0, n_slots * sizeof(RunDev)) != hipSuccess)
-    rc = fail(SGPE_E_HIP, "run state initialization failed");
-  if (!rc && hipMemset(p->d_inst, 0, n_slots * sizeof(InstDev)) != hipSuccess)
-    rc = fail(SGPE_E_HIP, "slot table initialization failed");
-  if (rc) {
-    sgpe_plan_destroy(p);
-    return rc;
-  }
-  *out = p;
-  return SGPE_OK;
+  return guarded([&]() -> int {
+    if (!ctx || !out || n_slots <= 0) return fail(SGPE_E_ARG, "bad arguments");
+    *out = nullptr;
+    if (precision != SGPE_C128 && precision != SGPE_C64) return fail(SGPE_E_ARG, "bad precision");
+    if (!pow2_in_range(nx) || !pow2_in_range(ny))
+      return fail(SGPE_E_UNSUPPORTED, "grid sizes must be powers of two in [64, 4096]");
+    if (n_slots > 65535) return fail(SGPE_E_ARG, "too many slots");
+    HIP_TRY(hipSetDevice(ctx->device));
+    sgpe_plan* p = new sgpe_plan();
+    p->ctx = ctx;
+    p->prec = precision;
+    p->ny = ny;
+    p->nx = nx;
+    p->nslots = n_slots;
+    p->row_stream = p->col_stream = n_slots == 1 ? 1 : 0;
+    p->h_inst.resize(n_slots);
+    p->slots.resize(n_slots);
+    int re, rr, ce, cw;
+    default_row(precision, nx, &re, &rr);
+    default_col(precision, ny, &ce, &cw);
+    int rc = plan_select(p, re, rr, ce, cw);
+    if (!rc && hipMalloc((void**)&p->d_inst, n_slots * sizeof(InstDev)) != hipSuccess)
+      rc = fail(SGPE_E_NOMEM, "slot table allocation failed");
+    if (!rc && hipMalloc((void**)&p->d_run, n_slots * sizeof(RunDev)) != hipSuccess)
+      rc = fail(SGPE_E_NOMEM, "run state allocation failed");
+    if (!rc && hipMemset(p->d_run, 0, n_slots * sizeof(RunDev)) != hipSuccess)
+      rc = fail(SGPE_E_HIP, "run state initialization failed");
+    if (!rc && hipMemset(p->d_inst, 0, n_slots * sizeof(InstDev)) != hipSuccess)
+      rc = fail(SGPE_E_HIP, "slot table initialization failed");
+    if (rc) {
+      sgpe_plan_destroy(p);
+      return rc;
+    }
+    *out = p;
+    return SGPE_OK;
+  });
 }
 
 void sgpe_plan_destroy(sgpe_plan* p) {
@@ -808,256 +842,290 @@ void sgpe_plan_destroy(sgpe_plan* p) {
 }
 
 int sgpe_plan_set_slot(sgpe_plan* p, int slot, const sgpe_inst_desc* d, void* psik) {
-  if (!p || !d) return fail(SGPE_E_ARG, "bad arguments");
-  if (slot < 0 || slot >= p->nslots) return fail(SGPE_E_ARG, "slot out of range");
-  HIP_TRY(hipSetDevice(p->ctx->device));
-  return plan_load_slot(p, slot, d, psik);
+  return guarded([&]() -> int {
+    if (!p || !d) return fail(SGPE_E_ARG, "bad arguments");
+    if (slot < 0 || slot >= p->nslots) return fail(SGPE_E_ARG, "slot out of range");
+    HIP_TRY(hipSetDevice(p->ctx->device));
+    return plan_load_slot(p, slot, d, psik);
+  });
 }
 
 int sgpe_plan_set_state(sgpe_plan* p, int slot, void* psik) {
-  if (!p || !psik || slot < 0 || slot >= p->nslots) return fail(SGPE_E_ARG, "bad arguments");
-  if (!p->slots[slot].loaded) return fail(SGPE_E_ARG, "slot has not been set");
-  HIP_TRY(hipSetDevice(p->ctx->device));
-  InstDev& h = p->h_inst[slot];
-  h.psi = psik;
-  HIP_TRY(hipMemcpyAsync(&p->d_inst[slot].psi, &h.psi, sizeof(void*), hipMemcpyHostToDevice, p->ctx->stream));
-  hipLaunchKernelGGL(config_run_kernel, dim3(1), dim3(64), 0, p->ctx->stream, p->d_run, slot, (double*)nullptr, 0, 1, 1,
-                     (int)FORM_KSPACE, 0.0);
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(hipStreamSynchronize(p->ctx->stream));
-  return SGPE_OK;
+  return guarded([&]() -> int {
+    if (!p || !psik || slot < 0 || slot >= p->nslots) return fail(SGPE_E_ARG, "bad arguments");
+    if (!p->slots[slot].loaded) return fail(SGPE_E_ARG, "slot has not been set");
+    HIP_TRY(hipSetDevice(p->ctx->device));
+    InstDev& h = p->h_inst[slot];
+    h.psi = psik;
+    HIP_TRY(hipMemcpyAsync(&p->d_inst[slot].psi, &h.psi, sizeof(void*), hipMemcpyHostToDevice, p->ctx->stream));
+    hipLaunchKernelGGL(config_run_kernel, dim3(1), dim3(64), 0, p->ctx->stream, p->d_run, slot, (double*)nullptr, 0, 1, 1,
+                       (int)FORM_KSPACE, 0.0);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(p->ctx->stream));
+    return SGPE_OK;
+  });
 }
 
 int sgpe_plan_set_run(sgpe_plan* p, int slot, double* pops, int cap, double conv_tol) {
-  if (!p || slot < 0 || slot >= p->nslots || cap < 0) return fail(SGPE_E_ARG, "bad arguments");
-  if (!p->slots[slot].loaded) return fail(SGPE_E_ARG, "slot has not been set");
-  HIP_TRY(hipSetDevice(p->ctx->device));
-  hipLaunchKernelGGL(config_run_kernel, dim3(1), dim3(64), 0, p->ctx->stream, p->d_run, slot, pops, pops ? cap : 0, 1,
-                     1, -1, conv_tol);
-  HIP_TRY(hipGetLastError());
-  return SGPE_OK;
+  return guarded([&]() -> int {
+    if (!p || slot < 0 || slot >= p->nslots || cap < 0) return fail(SGPE_E_ARG, "bad arguments");
+    if (!p->slots[slot].loaded) return fail(SGPE_E_ARG, "slot has not been set");
+    HIP_TRY(hipSetDevice(p->ctx->device));
+    hipLaunchKernelGGL(config_run_kernel, dim3(1), dim3(64), 0, p->ctx->stream, p->d_run, slot, pops, pops ? cap : 0, 1,
+                       1, -1, conv_tol);
+    HIP_TRY(hipGetLastError());
+    return SGPE_OK;
+  });
 }
 
 int sgpe_plan_status(sgpe_plan* p, int* out3, double* energy8) {
-  if (!p || !out3) return fail(SGPE_E_ARG, "bad arguments");
-  HIP_TRY(hipSetDevice(p->ctx->device));
-  std::vector<RunDev> h(p->nslots);
-  HIP_TRY(hipMemcpyAsync(h.data(), p->d_run, p->nslots * sizeof(RunDev), hipMemcpyDeviceToHost, p->ctx->stream));
-  HIP_TRY(hipStreamSynchronize(p->ctx->stream));
-  for (int i = 0; i < p->nslots; ++i) {
-    out3[3 * i + 0] = h[i].active;
-    out3[3 * i + 1] = h[i].converged;
-    out3[3 * i + 2] = h[i].step;
-    if (energy8)
-      for (int j = 0; j < 8; ++j) energy8[8 * i + j] = h[i].energy[j];
-  }
-  return SGPE_OK;
+  return guarded([&]() -> int {
+    if (!p || !out3) return fail(SGPE_E_ARG, "bad arguments");
+    HIP_TRY(hipSetDevice(p->ctx->device));
+    std::vector<RunDev> h(p->nslots);
+    HIP_TRY(hipMemcpyAsync(h.data(), p->d_run, p->nslots * sizeof(RunDev), hipMemcpyDeviceToHost, p->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(p->ctx->stream));
+    for (int i = 0; i < p->nslots; ++i) {
+      out3[3 * i + 0] = h[i].active;
+      out3[3 * i + 1] = h[i].converged;
+      out3[3 * i + 2] = h[i].step;
+      if (energy8)
+        for (int j = 0; j < 8; ++j) energy8[8 * i + j] = h[i].energy[j];
+    }
+    return SGPE_OK;
+  });
 }
 
 int sgpe_plan_set_active(sgpe_plan* p, int slot, int active) {
-  if (!p || slot < 0 || slot >= p->nslots) return fail(SGPE_E_ARG, "bad arguments");
-  if (active && !p->slots[slot].loaded) return fail(SGPE_E_ARG, "slot has not been set");
-  HIP_TRY(hipSetDevice(p->ctx->device));
-  hipLaunchKernelGGL(set_active_kernel, dim3(1), dim3(64), 0, p->ctx->stream, p->d_run, slot, active ? 1 : 0);
-  HIP_TRY(hipGetLastError());
-  return SGPE_OK;
+  return guarded([&]() -> int {
+    if (!p || slot < 0 || slot >= p->nslots) return fail(SGPE_E_ARG, "bad arguments");
+    if (active && !p->slots[slot].loaded) return fail(SGPE_E_ARG, "slot has not been set");
+    HIP_TRY(hipSetDevice(p->ctx->device));
+    hipLaunchKernelGGL(set_active_kernel, dim3(1), dim3(64), 0, p->ctx->stream, p->d_run, slot, active ? 1 : 0);
+    HIP_TRY(hipGetLastError());
+    return SGPE_OK;
+  });
 }
 
 int sgpe_list_shapes(int pass, int precision, int n, int* out_pairs, int cap) {
-  if ((pass != 0 && pass != 1) || (precision != SGPE_C128 && precision != SGPE_C64) || cap < 0)
-    return fail(SGPE_E_ARG, "bad arguments");
-  const KTable t = precision == SGPE_C128 ? registry_f64(pass == 0 ? K_ROW : K_COL)
-                                          : registry_f32(pass == 0 ? K_ROW : K_COL);
-  int count = 0;
-  for (int i = 0; i < t.count; ++i) {
-    if (t.entries[i].n != n) continue;
-    if (out_pairs && count < cap) {
-      out_pairs[2 * count] = t.entries[i].e;
-      out_pairs[2 * count + 1] = t.entries[i].p;
+  return guarded([&]() -> int {
+    if ((pass != 0 && pass != 1) || (precision != SGPE_C128 && precision != SGPE_C64) || cap < 0)
+      return fail(SGPE_E_ARG, "bad arguments");
+    const KTable t = precision == SGPE_C128 ? registry_f64(pass == 0 ? K_ROW : K_COL)
+                                            : registry_f32(pass == 0 ? K_ROW : K_COL);
+    int count = 0;
+    for (int i = 0; i < t.count; ++i) {
+      if (t.entries[i].n != n) continue;
+      if (out_pairs && count < cap) {
+        out_pairs[2 * count] = t.entries[i].e;
+        out_pairs[2 * count + 1] = t.entries[i].p;
+      }
+      ++count;
     }
-    ++count;
-  }
-  return count;
+    return count;
+  });
 }
 
 int sgpe_plan_tune(sgpe_plan* p, int row_e, int row_rows, int col_e, int col_w) {
-  if (!p) return fail(SGPE_E_ARG, "plan is NULL");
-  HIP_TRY(hipSetDevice(p->ctx->device));
-  return plan_select(p, row_e ? row_e : p->row_e, row_rows ? row_rows : p->row_rows, col_e ? col_e : p->col_e,
-                     col_w ? col_w : p->col_w);
+  return guarded([&]() -> int {
+    if (!p) return fail(SGPE_E_ARG, "plan is NULL");
+    HIP_TRY(hipSetDevice(p->ctx->device));
+    return plan_select(p, row_e ? row_e : p->row_e, row_rows ? row_rows : p->row_rows, col_e ? col_e : p->col_e,
+                       col_w ? col_w : p->col_w);
+  });
 }
 
 int sgpe_plan_set_streaming(sgpe_plan* p, int row_stream, int col_stream) {
-  if (!p) return fail(SGPE_E_ARG, "plan is NULL");
-  p->row_stream = row_stream ? 1 : 0;
-  p->col_stream = col_stream ? 1 : 0;
-  drop_graphs(p);
-  return SGPE_OK;
+  return guarded([&]() -> int {
+    if (!p) return fail(SGPE_E_ARG, "plan is NULL");
+    p->row_stream = row_stream ? 1 : 0;
+    p->col_stream = col_stream ? 1 : 0;
+    drop_graphs(p);
+    return SGPE_OK;
+  });
 }
 
 int sgpe_plan_get_streaming(const sgpe_plan* p, int* out2) {
-  if (!p || !out2) return fail(SGPE_E_ARG, "bad arguments");
-  out2[0] = p->row_stream;
-  out2[1] = p->col_stream;
-  return SGPE_OK;
+  return guarded([&]() -> int {
+    if (!p || !out2) return fail(SGPE_E_ARG, "bad arguments");
+    out2[0] = p->row_stream;
+    out2[1] = p->col_stream;
+    return SGPE_OK;
+  });
 }
 
 int sgpe_plan_get_tuning(const sgpe_plan* p, int* out4) {
-  if (!p || !out4) return fail(SGPE_E_ARG, "bad arguments");
-  out4[0] = p->row_e;
-  out4[1] = p->row_rows;
-  out4[2] = p->col_e;
-  out4[3] = p->col_w;
-  return SGPE_OK;
+  return guarded([&]() -> int {
+    if (!p || !out4) return fail(SGPE_E_ARG, "bad arguments");
+    out4[0] = p->row_e;
+    out4[1] = p->row_rows;
+    out4[2] = p->col_e;
+    out4[3] = p->col_w;
+    return SGPE_OK;
+  });
 }
 
 int sgpe_single_step(sgpe_plan* p, int which) {
-  int rc = plan_ready(p);
-  if (rc) return rc;
-  if (which != SGPE_SUB_OUTER && which != SGPE_SUB_INNER) return fail(SGPE_E_ARG, "bad sub-step");
-  HIP_TRY(hipSetDevice(p->ctx->device));
-  issue_sub_step(p, p->ctx->stream, which, 0);
-  HIP_TRY(hipGetLastError());
-  return SGPE_OK;
+  return guarded([&]() -> int {
+    int rc = plan_ready(p);
+    if (rc) return rc;
+    if (which != SGPE_SUB_OUTER && which != SGPE_SUB_INNER) return fail(SGPE_E_ARG, "bad sub-step");
+    HIP_TRY(hipSetDevice(p->ctx->device));
+    issue_sub_step(p, p->ctx->stream, which, 0);
+    HIP_TRY(hipGetLastError());
+    return SGPE_OK;
+  });
 }
 
 int sgpe_full_steps(sgpe_plan* p, int n_steps) {
-  int rc = plan_ready(p);
-  if (rc) return rc;
-  if (n_steps < 0) return fail(SGPE_E_ARG, "n_steps < 0");
-  HIP_TRY(hipSetDevice(p->ctx->device));
-  for (int i = 0; i < n_steps; ++i) issue_full_step(p, p->ctx->stream);
-  HIP_TRY(hipGetLastError());
-  return SGPE_OK;
+  return guarded([&]() -> int {
+    int rc = plan_ready(p);
+    if (rc) return rc;
+    if (n_steps < 0) return fail(SGPE_E_ARG, "n_steps < 0");
+    HIP_TRY(hipSetDevice(p->ctx->device));
+    for (int i = 0; i < n_steps; ++i) issue_full_step(p, p->ctx->stream);
+    HIP_TRY(hipGetLastError());
+    return SGPE_OK;
+  });
 }
 
 int sgpe_graph_steps(sgpe_plan* p, int k, int n_graphs) {
-  int rc = plan_ready(p);
-  if (rc) return rc;
-  if (k <= 0 || n_graphs < 0) return fail(SGPE_E_ARG, "bad step counts");
-  HIP_TRY(hipSetDevice(p->ctx->device));
-  if (n_graphs == 0) return SGPE_OK;
-  // Captured blocks assume the pending closing half-step is an outer one.
-  if (p->last_dur != DUR_OUT) issue_sync(p, p->ctx->stream);
-  auto it = p->graphs.find(k);
-  if (it == p->graphs.end()) {
-    hipGraph_t g = nullptr;
-    hipGraphExec_t ge = nullptr;
-    hipStream_t cs = p->ctx->cap_stream;
-    HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < k; ++i) issue_full_step(p, cs);
-    hipError_t le = hipGetLastError();
-    hipError_t ce = hipStreamEndCapture(cs, &g);
-    if (le != hipSuccess || ce != hipSuccess) {
-      if (g) (void)hipGraphDestroy(g);
-      return fail(SGPE_E_HIP, std::string("graph capture failed: ") + hipGetErrorString(le != hipSuccess ? le : ce));
+  return guarded([&]() -> int {
+    int rc = plan_ready(p);
+    if (rc) return rc;
+    if (k <= 0 || n_graphs < 0) return fail(SGPE_E_ARG, "bad step counts");
+    HIP_TRY(hipSetDevice(p->ctx->device));
+    if (n_graphs == 0) return SGPE_OK;
+    // Captured blocks assume the pending closing half-step is an outer one.
+    if (p->last_dur != DUR_OUT) issue_sync(p, p->ctx->stream);
+    auto it = p->graphs.find(k);
+    if (it == p->graphs.end()) {
+      hipGraph_t g = nullptr;
+      hipGraphExec_t ge = nullptr;
+      hipStream_t cs = p->ctx->cap_stream;
+      HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+      for (int i = 0; i < k; ++i) issue_full_step(p, cs);
+      hipError_t le = hipGetLastError();
+      hipError_t ce = hipStreamEndCapture(cs, &g);
+      if (le != hipSuccess || ce != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(SGPE_E_HIP, std::string("graph capture failed: ") + hipGetErrorString(le != hipSuccess ? le : ce));
+      }
+      if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(SGPE_E_HIP, "graph instantiation failed");
+      }
+      p->graph_defs[k] = g;
+      it = p->graphs.emplace(k, ge).first;
     }
-    if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
-      (void)hipGraphDestroy(g);
-      return fail(SGPE_E_HIP, "graph instantiation failed");
-    }
-    p->graph_defs[k] = g;
-    it = p->graphs.emplace(k, ge).first;
-  }
-  for (int i = 0; i < n_graphs; ++i) HIP_TRY(hipGraphLaunch(it->second, p->ctx->stream));
-  p->last_dur = DUR_OUT;
-  return SGPE_OK;
+    for (int i = 0; i < n_graphs; ++i) HIP_TRY(hipGraphLaunch(it->second, p->ctx->stream));
+    p->last_dur = DUR_OUT;
+    return SGPE_OK;
+  });
 }
 
 int sgpe_profile_full_steps(sgpe_plan* p, int n_steps, double* out4) {
-  int rc = plan_ready(p);
-  if (rc) return rc;
-  if (n_steps <= 0 || !out4) return fail(SGPE_E_ARG, "bad arguments");
-  HIP_TRY(hipSetDevice(p->ctx->device));
-  hipStream_t s = p->ctx->stream;
-  const int n_launch = 6 * n_steps;
-  std::vector<hipEvent_t> ev(2 * n_launch, nullptr);
-  for (auto& e : ev)
-    if (hipEventCreate(&e) != hipSuccess) {
-      for (auto& f : ev)
-        if (f) (void)hipEventDestroy(f);
-      return fail(SGPE_E_HIP, "event creation failed");
+  return guarded([&]() -> int {
+    int rc = plan_ready(p);
+    if (rc) return rc;
+    if (n_steps <= 0 || !out4) return fail(SGPE_E_ARG, "bad arguments");
+    HIP_TRY(hipSetDevice(p->ctx->device));
+    hipStream_t s = p->ctx->stream;
+    const int n_launch = 6 * n_steps;
+    std::vector<hipEvent_t> ev(2 * n_launch, nullptr);
+    for (auto& e : ev)
+      if (hipEventCreate(&e) != hipSuccess) {
+        for (auto& f : ev)
+          if (f) (void)hipEventDestroy(f);
+        return fail(SGPE_E_HIP, "event creation failed");
+      }
+    // The launches of sgpe_full_steps, each carrying a start/stop event pair
+    // that timestamps the kernel itself (no extra packets between kernels).
+    int k = 0;
+    for (int i = 0; i < n_steps; ++i) {
+      const int durs[3] = {DUR_OUT, DUR_IN, DUR_OUT};
+      for (int j = 0; j < 3; ++j) {
+        launch_col(p, s, COL_MID, p->last_dur, durs[j], ev[2 * k], ev[2 * k + 1]);
+        ++k;
+        launch_row(p, s, durs[j], j == 0, ev[2 * k], ev[2 * k + 1]);
+        ++k;
+        p->last_dur = durs[j];
+      }
     }
-  // The launches of sgpe_full_steps, each carrying a start/stop event pair
-  // that timestamps the kernel itself (no extra packets between kernels).
-  int k = 0;
-  for (int i = 0; i < n_steps; ++i) {
-    const int durs[3] = {DUR_OUT, DUR_IN, DUR_OUT};
-    for (int j = 0; j < 3; ++j) {
-      launch_col(p, s, COL_MID, p->last_dur, durs[j], ev[2 * k], ev[2 * k + 1]);
-      ++k;
-      launch_row(p, s, durs[j], j == 0, ev[2 * k], ev[2 * k + 1]);
-      ++k;
-      p->last_dur = durs[j];
+    hipError_t le = hipGetLastError();
+    hipError_t se = hipStreamSynchronize(s);
+    double col = 0.0, row = 0.0;
+    for (int i = 0; i < n_launch && le == hipSuccess && se == hipSuccess; ++i) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]) != hipSuccess) break;
+      if (i % 2 == 0)
+        col += ms;
+      else
+        row += ms;
     }
-  }
-  hipError_t le = hipGetLastError();
-  hipError_t se = hipStreamSynchronize(s);
-  double col = 0.0, row = 0.0;
-  for (int i = 0; i < n_launch && le == hipSuccess && se == hipSuccess; ++i) {
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]) != hipSuccess) break;
-    if (i % 2 == 0)
-      col += ms;
-    else
-      row += ms;
-  }
-  for (auto& e : ev) (void)hipEventDestroy(e);
-  if (le != hipSuccess) return fail(SGPE_E_HIP, std::string("launch failed: ") + hipGetErrorString(le));
-  if (se != hipSuccess) return fail(SGPE_E_HIP, std::string("synchronize failed: ") + hipGetErrorString(se));
-  out4[0] = col;
-  out4[1] = row;
-  out4[2] = 3.0 * n_steps;
-  out4[3] = 3.0 * n_steps;
-  return SGPE_OK;
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    if (le != hipSuccess) return fail(SGPE_E_HIP, std::string("launch failed: ") + hipGetErrorString(le));
+    if (se != hipSuccess) return fail(SGPE_E_HIP, std::string("synchronize failed: ") + hipGetErrorString(se));
+    out4[0] = col;
+    out4[1] = row;
+    out4[2] = 3.0 * n_steps;
+    out4[3] = 3.0 * n_steps;
+    return SGPE_OK;
+  });
 }
 
 int sgpe_plan_sync(sgpe_plan* p) {
-  int rc = plan_ready(p);
-  if (rc) return rc;
-  HIP_TRY(hipSetDevice(p->ctx->device));
-  issue_sync(p, p->ctx->stream);
-  HIP_TRY(hipGetLastError());
-  return SGPE_OK;
+  return guarded([&]() -> int {
+    int rc = plan_ready(p);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(p->ctx->device));
+    issue_sync(p, p->ctx->stream);
+    HIP_TRY(hipGetLastError());
+    return SGPE_OK;
+  });
 }
 
 int sgpe_energy(sgpe_plan* p, double* out) {
-  int rc = plan_ready(p);
-  if (rc) return rc;
-  HIP_TRY(hipSetDevice(p->ctx->device));
-  if ((rc = plan_energy_prepare(p))) return rc;
-  issue_energy(p, p->ctx->stream, out);
-  HIP_TRY(hipGetLastError());
-  return SGPE_OK;
+  return guarded([&]() -> int {
+    int rc = plan_ready(p);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(p->ctx->device));
+    if ((rc = plan_energy_prepare(p))) return rc;
+    issue_energy(p, p->ctx->stream, out);
+    HIP_TRY(hipGetLastError());
+    return SGPE_OK;
+  });
 }
 
 int sgpe_group_create(sgpe_ctx* ctx, int n_plans, sgpe_plan* const* plans, sgpe_group** out) {
-  if (!ctx || !out || n_plans <= 0 || !plans) return fail(SGPE_E_ARG, "bad arguments");
-  *out = nullptr;
-  for (int i = 0; i < n_plans; ++i) {
-    if (!plans[i] || plans[i]->ctx != ctx) return fail(SGPE_E_ARG, "plans must be non-NULL and share the context");
-    for (int j = 0; j < i; ++j)
-      if (plans[j] == plans[i]) return fail(SGPE_E_ARG, "a plan appears twice");
-  }
-  HIP_TRY(hipSetDevice(ctx->device));
-  sgpe_group* g = new sgpe_group();
-  g->ctx = ctx;
-  g->plans.assign(plans, plans + n_plans);
-  bool ok = hipEventCreateWithFlags(&g->fork, hipEventDisableTiming) == hipSuccess;
-  for (int i = 0; i < n_plans && ok; ++i) {
-    hipStream_t st = nullptr;
-    hipEvent_t ev = nullptr;
-    ok = hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess;
-    if (ok) g->side.push_back(st);
-    ok = ok && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
-    if (ok) g->join.push_back(ev);
-  }
-  if (!ok) {
-    sgpe_group_destroy(g);
-    return fail(SGPE_E_HIP, "group stream/event creation failed");
-  }
-  *out = g;
-  return SGPE_OK;
+  return guarded([&]() -> int {
+    if (!ctx || !out || n_plans <= 0 || !plans) return fail(SGPE_E_ARG, "bad arguments");
+    *out = nullptr;
+    for (int i = 0; i < n_plans; ++i) {
+      if (!plans[i] || plans[i]->ctx != ctx) return fail(SGPE_E_ARG, "plans must be non-NULL and share the context");
+      for (int j = 0; j < i; ++j)
+        if (plans[j] == plans[i]) return fail(SGPE_E_ARG, "a plan appears twice");
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    sgpe_group* g = new sgpe_group();
+    g->ctx = ctx;
+    g->plans.assign(plans, plans + n_plans);
+    bool ok = hipEventCreateWithFlags(&g->fork, hipEventDisableTiming) == hipSuccess;
+    for (int i = 0; i < n_plans && ok; ++i) {
+      hipStream_t st = nullptr;
+      hipEvent_t ev = nullptr;
+      ok = hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess;
+      if (ok) g->side.push_back(st);
+      ok = ok && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
+      if (ok) g->join.push_back(ev);
+    }
+    if (!ok) {
+      sgpe_group_destroy(g);
+      return fail(SGPE_E_HIP, "group stream/event creation failed");
+    }
+    *out = g;
+    return SGPE_OK;
+  });
 }
 
 void sgpe_group_destroy(sgpe_group* g) {
@@ -1073,62 +1141,64 @@ void sgpe_group_destroy(sgpe_group* g) {
 }
 
 int sgpe_group_run(sgpe_group* g, int k, int check, int n_graphs) {
-  if (!g || k <= 0 || n_graphs < 0) return fail(SGPE_E_ARG, "bad arguments");
-  HIP_TRY(hipSetDevice(g->ctx->device));
-  int rc;
-  for (auto* p : g->plans) {
-    if ((rc = plan_ready(p))) return rc;
-    if (check && (rc = plan_energy_prepare(p))) return rc;
-    if (p->last_dur != DUR_OUT) issue_sync(p, g->ctx->stream);
-  }
-  if (n_graphs == 0) return SGPE_OK;
-  // A plan re-tuned, re-streamed or switched kernel variants since the
-  // capture: its kernels in the cached graphs are stale.
-  std::vector<unsigned> now;
-  for (auto* p : g->plans) now.push_back(p->epoch);
-  if (now != g->epochs) {
-    for (auto& x : g->graphs) (void)hipGraphExecDestroy(x.second);
-    for (auto& x : g->defs) (void)hipGraphDestroy(x.second);
-    g->graphs.clear();
-    g->defs.clear();
-    g->epochs = now;
-  }
-  const long key = 2L * k + (check ? 1 : 0);
-  auto it = g->graphs.find(key);
-  if (it == g->graphs.end()) {
-    // One capture: the context stream forks into one branch per plan, each
-    // branch runs k full steps (and the energy/convergence check), and the
-    // branches join back, so different grid sizes run concurrently.
-    hipStream_t cs = g->ctx->cap_stream;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    hipError_t e = hipEventRecord(g->fork, cs);
-    for (size_t i = 0; i < g->plans.size() && e == hipSuccess; ++i) {
-      e = hipStreamWaitEvent(g->side[i], g->fork, 0);
-      if (e != hipSuccess) break;
-      for (int j = 0; j < k; ++j) issue_full_step(g->plans[i], g->side[i]);
-      if (check) issue_energy(g->plans[i], g->side[i], nullptr);
-      e = hipEventRecord(g->join[i], g->side[i]);
-      if (e == hipSuccess) e = hipStreamWaitEvent(cs, g->join[i], 0);
+  return guarded([&]() -> int {
+    if (!g || k <= 0 || n_graphs < 0) return fail(SGPE_E_ARG, "bad arguments");
+    HIP_TRY(hipSetDevice(g->ctx->device));
+    int rc;
+    for (auto* p : g->plans) {
+      if ((rc = plan_ready(p))) return rc;
+      if (check && (rc = plan_energy_prepare(p))) return rc;
+      if (p->last_dur != DUR_OUT) issue_sync(p, g->ctx->stream);
     }
-    hipError_t le = hipGetLastError();
-    hipError_t ce = hipStreamEndCapture(cs, &graph);
-    if (e != hipSuccess || le != hipSuccess || ce != hipSuccess) {
-      if (graph) (void)hipGraphDestroy(graph);
-      hipError_t bad = e != hipSuccess ? e : (le != hipSuccess ? le : ce);
-      return fail(SGPE_E_HIP, std::string("group graph capture failed: ") + hipGetErrorString(bad));
+    if (n_graphs == 0) return SGPE_OK;
+    // A plan re-tuned, re-streamed or switched kernel variants since the
+    // capture: its kernels in the cached graphs are stale.
+    std::vector<unsigned> now;
+    for (auto* p : g->plans) now.push_back(p->epoch);
+    if (now != g->epochs) {
+      for (auto& x : g->graphs) (void)hipGraphExecDestroy(x.second);
+      for (auto& x : g->defs) (void)hipGraphDestroy(x.second);
+      g->graphs.clear();
+      g->defs.clear();
+      g->epochs = now;
     }
-    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
-      (void)hipGraphDestroy(graph);
-      return fail(SGPE_E_HIP, "group graph instantiation failed");
+    const long key = 2L * k + (check ? 1 : 0);
+    auto it = g->graphs.find(key);
+    if (it == g->graphs.end()) {
+      // One capture: the context stream forks into one branch per plan, each
+      // branch runs k full steps (and the energy/convergence check), and the
+      // branches join back, so different grid sizes run concurrently.
+      hipStream_t cs = g->ctx->cap_stream;
+      hipGraph_t graph = nullptr;
+      hipGraphExec_t exec = nullptr;
+      HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+      hipError_t e = hipEventRecord(g->fork, cs);
+      for (size_t i = 0; i < g->plans.size() && e == hipSuccess; ++i) {
+        e = hipStreamWaitEvent(g->side[i], g->fork, 0);
+        if (e != hipSuccess) break;
+        for (int j = 0; j < k; ++j) issue_full_step(g->plans[i], g->side[i]);
+        if (check) issue_energy(g->plans[i], g->side[i], nullptr);
+        e = hipEventRecord(g->join[i], g->side[i]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(cs, g->join[i], 0);
+      }
+      hipError_t le = hipGetLastError();
+      hipError_t ce = hipStreamEndCapture(cs, &graph);
+      if (e != hipSuccess || le != hipSuccess || ce != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        hipError_t bad = e != hipSuccess ? e : (le != hipSuccess ? le : ce);
+        return fail(SGPE_E_HIP, std::string("group graph capture failed: ") + hipGetErrorString(bad));
+      }
+      if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return fail(SGPE_E_HIP, "group graph instantiation failed");
+      }
+      g->defs[key] = graph;
+      it = g->graphs.emplace(key, exec).first;
     }
-    g->defs[key] = graph;
-    it = g->graphs.emplace(key, exec).first;
-  }
-  for (int i = 0; i < n_graphs; ++i) HIP_TRY(hipGraphLaunch(it->second, g->ctx->stream));
-  for (auto* p : g->plans) p->last_dur = DUR_OUT;
-  return SGPE_OK;
+    for (int i = 0; i < n_graphs; ++i) HIP_TRY(hipGraphLaunch(it->second, g->ctx->stream));
+    for (auto* p : g->plans) p->last_dur = DUR_OUT;
+    return SGPE_OK;
+  });
 }
 
 }  // extern "C"

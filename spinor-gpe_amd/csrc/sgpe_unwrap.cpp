@@ -20,6 +20,8 @@
 #include <cstdint>
 #include <vector>
 
+#include "../../include/sgpe.h"
+
 namespace {
 
 const double kTwoPi = 6.283185307179586476925;
@@ -45,8 +47,7 @@ struct Edge {
 
 }  // namespace
 
-extern "C" int sgpe_unwrap_phase_2d(const double* ph, double* out, int ny, int nx) {
-  if (!ph || !out || ny < 1 || nx < 1) return -1;
+static int unwrap_2d(const double* ph, double* out, int ny, int nx) {
   const int n = ny * nx;
   std::vector<double> rel(n, 1.0e7);
   for (int i = 1; i + 1 < ny; ++i)
@@ -99,4 +100,13 @@ extern "C" int sgpe_unwrap_phase_2d(const double* ph, double* out, int ny, int n
   }
   for (int p = 0; p < n; ++p) out[p] = ph[p] + kTwoPi * turns[p];
   return 0;
+}
+
+extern "C" int sgpe_unwrap_phase_2d(const double* ph, double* out, int ny, int nx) {
+  if (!ph || !out || ny < 1 || nx < 1 || (int64_t)ny * nx > INT32_MAX) return SGPE_E_ARG;
+  try {
+    return unwrap_2d(ph, out, ny, nx);
+  } catch (...) {  // std::bad_alloc of the work arrays: no exception crosses the C ABI
+    return SGPE_E_NOMEM;
+  }
 }

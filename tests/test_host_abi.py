@@ -117,6 +117,19 @@ def test_native_unwrap_ramp_and_identity():
     np.testing.assert_array_equal(nat.unwrap_phase(small), small)
 
 
+def test_native_unwrap_rejects_bad_arguments():
+    """Argument errors come back as SGPE_E_ARG without touching the arrays:
+    null pointers, empty grids, and grids whose ny * nx overflows int."""
+    import ctypes  # pylint: disable=import-outside-toplevel
+    buf = np.zeros(4)
+    ptr = buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    f = nat.lib().sgpe_unwrap_phase_2d
+    assert f(None, ptr, 2, 2) == -1
+    assert f(ptr, ptr, 0, 2) == -1
+    assert f(ptr, ptr, 50000, 50000) == -1
+    assert f(ptr, buf[2:].ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 1, 2) == 0
+
+
 def test_precision_codes():
     for v in ('c128', 'complex128', nat.C128, __import__('torch').complex128):
         assert nat.precision_of(v) == nat.C128
