@@ -9,7 +9,6 @@ arithmetic, float64 reductions): 2e-4 and 1e-5.
 
 import numpy as np
 import pytest
-import torch
 
 from cases import CASES
 from oracle import energy as oe

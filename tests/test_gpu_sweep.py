@@ -6,7 +6,6 @@ import numpy as np
 import pytest
 
 from cases import base, kl3_scale, WAVEL
-from spinor_gpe.engine.plan import InstanceSpec
 from spinor_gpe.engine.sweep import Sweep
 
 pytestmark = pytest.mark.gpu
