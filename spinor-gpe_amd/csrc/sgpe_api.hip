@@ -296,12 +296,21 @@ static int plan_select(sgpe_plan* p, int row_e, int row_rows, int col_e, int col
   return SGPE_OK;
 }
 
+// State address handed to the pass kernels as an argument: for a plan of
+// one loaded slot (the kernels then issue their state loads before reading
+// the slot's run flags). Captured graphs hold it, so rebinding the state
+// drops them.
+static void* direct_state(const sgpe_plan* p) {
+  return p->nslots == 1 && p->slots[0].loaded ? p->h_inst[0].psi : nullptr;
+}
+
 static void launch_col(sgpe_plan* p, hipStream_t s, int mode, int dprev, int dcur, hipEvent_t e0 = nullptr,
                        hipEvent_t e1 = nullptr) {
   const size_t lds = col_lds_bytes(p->col_w, p->ny, csize(p->prec));
   dim3 grid(p->nx / p->col_w, 2, p->nslots), block(p->col_w * (p->ny / p->col_e));
   ((ColFn)(p->dense_kin ? p->col->launch_g : p->col->launch))(grid, block, lds, s, e0, e1, p->d_inst, p->d_run,
-                                                              p->tw_y, mode, dprev, dcur, p->col_stream);
+                                                              direct_state(p), p->nx, p->tw_y, mode, dprev, dcur,
+                                                              p->col_stream);
 }
 
 static void launch_row(sgpe_plan* p, hipStream_t s, int dur, int first, hipEvent_t e0 = nullptr,
@@ -309,7 +318,8 @@ static void launch_row(sgpe_plan* p, hipStream_t s, int dur, int first, hipEvent
   const size_t lds = row_lds_bytes(p->row_e, (size_t)p->row_rows * 2 * p->nx * csize(p->prec), p->nx, csize(p->prec));
   dim3 grid(p->ny / p->row_rows, 1, p->nslots), block(p->row_rows * (p->nx / p->row_e));
   ((RowFn)(p->general_row ? p->row->launch_g : p->row->launch))(grid, block, lds, s, e0, e1, p->d_inst, p->d_run,
-                                                                p->tw_x, dur, first, p->col_w, p->row_stream);
+                                                                direct_state(p), p->ny, p->tw_x, dur, first, p->col_w,
+                                                                p->row_stream);
 }
 
 static void issue_sub_step(sgpe_plan* p, hipStream_t s, int which, int first) {
@@ -608,7 +618,7 @@ static int plan_load_slot(sgpe_plan* p, int slot, const sgpe_inst_desc* d, void*
     dk = dk || q.t_dense[0] || q.t_dense[1];
     gr = gr || q.v_dense[0] || q.v_dense[1] || q.coupling == SGPE_COUPLING_ARRAY;
   }
-  if (dk != p->dense_kin || gr != p->general_row) {
+  if (dk != p->dense_kin || gr != p->general_row || p->nslots == 1) {  // one slot: graphs hold its state address
     p->dense_kin = dk;
     p->general_row = gr;
     drop_graphs(p);
@@ -857,6 +867,7 @@ int sgpe_plan_set_state(sgpe_plan* p, int slot, void* psik) {
     HIP_TRY(hipSetDevice(p->ctx->device));
     InstDev& h = p->h_inst[slot];
     h.psi = psik;
+    drop_graphs(p);  // captured single-slot passes hold the state address
     HIP_TRY(hipMemcpyAsync(&p->d_inst[slot].psi, &h.psi, sizeof(void*), hipMemcpyHostToDevice, p->ctx->stream));
     hipLaunchKernelGGL(config_run_kernel, dim3(1), dim3(64), 0, p->ctx->stream, p->d_run, slot, (double*)nullptr, 0, 1, 1,
                        (int)FORM_KSPACE, 0.0);

@@ -226,8 +226,8 @@ __device__ __forceinline__ void col_kinetic(C (&a)[E], int t, int nx, int jx, co
 // table slots); otherwise every slot uses the separable tables.
 template <typename R, int N, int E, int W, bool DENSE>
 __global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __restrict__ insts,
-                                                             RunDev* __restrict__ runs,
-                                                             const typename Cx<R>::T* __restrict__ tws,
+                                                             RunDev* __restrict__ runs, void* direct_psi,
+                                                             int direct_n, const typename Cx<R>::T* __restrict__ tws,
                                                              int mode, int d_prev, int d_cur, int stream) {
   typedef typename Cx<R>::T C;
   typedef LdsView<W, 1 << 30, 0> V;
@@ -242,14 +242,17 @@ __global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __re
   extern __shared__ double smem[];
   const int inst = blockIdx.z;
   const InstDev& P = insts[inst];
-  // The state's address and row length are read together with the active
-  // flag, so the state loads wait for one round of scalar loads, not two.
-  C* const psi = reinterpret_cast<C*>(P.psi);
-  const int nx = P.nx;
-  if (!runs[inst].active || !psi || nx <= 0) return;
-  const bool kspace = runs[inst].form == FORM_KSPACE;
+  // A single-slot plan passes its state's address and row length as kernel
+  // arguments (direct_psi): the state loads are issued at once and the
+  // active flag is checked after them. Otherwise the address and length are
+  // read together with the active flag, so the state loads wait for one
+  // round of scalar loads, not two.
+  const bool direct = direct_psi != nullptr;
+  C* const psi = reinterpret_cast<C*>(direct ? direct_psi : P.psi);
+  const int nx = direct ? direct_n : P.nx;
+  if (!direct && (!runs[inst].active || !psi || nx <= 0)) return;
   const int comp = blockIdx.y;
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tile = xcd_tile(blockIdx.x, nx / W);
   const int c = threadIdx.x % W;
   const int t = threadIdx.x / W;
   const int jx = tile * W + c;
@@ -282,6 +285,14 @@ __global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __re
   sp.load(td ? tws : reinterpret_cast<const C*>(P.ky[d_prev][comp]));
   sc_.load(td ? tws : reinterpret_cast<const C*>(P.ky[d_cur][comp]));
   stw.load(tws);
+  if (direct) {
+    // The prologue loads stay ahead of the flag test (an inactive slot's
+    // state and tables are still allocated: the host passes direct_psi only
+    // for a loaded slot).
+    asm volatile("" ::: "memory");
+    if (!runs[inst].active) return;
+  }
+  const bool kspace = runs[inst].form == FORM_KSPACE;
   const C* kyp = sp.store(sm + W * N + N);
   const C* kyc = sc_.store(sm + W * N + 2 * N);
   const C* tw = stw.store(sm + W * N);
@@ -318,7 +329,7 @@ __global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __re
   else
     block_sum<2>(v, smem);
   if (threadIdx.x == 0) {
-    const int nt = P.nx / W;
+    const int nt = nx / W;
     P.part[PART_S * 2 * nt + comp * nt + tile] = v[0];
     P.part[PART_NRM * 2 * nt + comp * nt + tile] = v[1];
   }
@@ -470,7 +481,8 @@ __device__ __forceinline__ void pointwise_points(C (&a0)[E], C (&a1)[E], const I
 // every slot takes the separable fast path.
 template <typename R, int N, int E, int ROWS, bool GENERAL>
 __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* __restrict__ insts,
-                                                                RunDev* __restrict__ runs,
+                                                                RunDev* __restrict__ runs, void* direct_psi,
+                                                                int direct_n,
                                                                 const typename Cx<R>::T* __restrict__ tws,
                                                                 int dur, int first_sub, int col_w, int stream) {
   typedef typename Cx<R>::T C;
@@ -486,11 +498,13 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
   const int inst = blockIdx.z;
   RunDev& run = runs[inst];
   const InstDev& P = insts[inst];
-  // The state's address and size are read together with the active flag,
-  // so the state loads wait for one round of scalar loads, not two.
-  C* const psi = reinterpret_cast<C*>(P.psi);
-  const int ny = P.ny;
-  if (!run.active || !psi || ny <= 0) return;
+  // State address and column length: kernel arguments for a single-slot
+  // plan (loads first, active flag checked after them), else read together
+  // with the active flag (one round of scalar loads before the state loads).
+  const bool direct = direct_psi != nullptr;
+  C* const psi = reinterpret_cast<C*>(direct ? direct_psi : P.psi);
+  const int ny = direct ? direct_n : P.ny;
+  if (!direct && (!run.active || !psi || ny <= 0)) return;
   const int nt2 = 2 * (N / col_w);
   const int row = threadIdx.x / T;
   const int t = threadIdx.x % T;
@@ -538,6 +552,12 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
   if constexpr (kEarlyTables) load_x_tables<E, T>(tx0, tx1, tex, px0, px1, eth, tws, t);
   TableStage<kTw, N, kBlock, C> stw;
   stw.load(tws);
+  if (direct) {
+    // The prologue loads stay ahead of the flag test (a loaded slot's state
+    // and tables stay allocated when it is inactive).
+    asm volatile("" ::: "memory");
+    if (!run.active) return;
+  }
 
   // Real-space normalization (tensor_tools.py:301-305) from the column
   // pass's k-space sums by Parseval: sum_r |IDFT2 v|^2 = Nx Ny sum_k |v|^2.

@@ -11,10 +11,12 @@ namespace sgpe {
 
 // Split-step pass launchers; with non-null events the launch carries them
 // (hipExtLaunchKernelGGL), which timestamps the kernel's own execution.
-typedef void (*ColFn)(dim3, dim3, size_t, hipStream_t, hipEvent_t, hipEvent_t, const InstDev*, RunDev*, const void*,
-                      int, int, int, int);
-typedef void (*RowFn)(dim3, dim3, size_t, hipStream_t, hipEvent_t, hipEvent_t, const InstDev*, RunDev*, const void*,
-                      int, int, int, int);
+// The void*/int pair is the direct state address and length (null: read
+// from the slot table).
+typedef void (*ColFn)(dim3, dim3, size_t, hipStream_t, hipEvent_t, hipEvent_t, const InstDev*, RunDev*, void*, int,
+                      const void*, int, int, int, int);
+typedef void (*RowFn)(dim3, dim3, size_t, hipStream_t, hipEvent_t, hipEvent_t, const InstDev*, RunDev*, void*, int,
+                      const void*, int, int, int, int);
 typedef void (*FRowFn)(dim3, dim3, size_t, hipStream_t, const void*, void*, int, int, double, const void*);
 typedef void (*FColFn)(dim3, dim3, size_t, hipStream_t, const void*, void*, int, int, const void*);
 typedef void (*EFn)(dim3, dim3, size_t, hipStream_t, const InstDev*, const RunDev*, const void*);
@@ -39,23 +41,24 @@ KTable registry_f32(int kind);
 
 template <typename R, int N, int E, int W, bool G>
 void col_launch(dim3 g, dim3 b, size_t l, hipStream_t s, hipEvent_t e0, hipEvent_t e1, const InstDev* i, RunDev* r,
-                const void* tw, int m, int dp, int dc, int st) {
+                void* dpsi, int dn, const void* tw, int m, int dp, int dc, int st) {
   typedef typename Cx<R>::T C;
   if (e0)
-    hipExtLaunchKernelGGL((col_pass_kernel<R, N, E, W, G>), g, b, (uint32_t)l, s, e0, e1, 0, i, r, (const C*)tw, m,
-                          dp, dc, st);
+    hipExtLaunchKernelGGL((col_pass_kernel<R, N, E, W, G>), g, b, (uint32_t)l, s, e0, e1, 0, i, r, dpsi, dn,
+                          (const C*)tw, m, dp, dc, st);
   else
-    hipLaunchKernelGGL((col_pass_kernel<R, N, E, W, G>), g, b, l, s, i, r, (const C*)tw, m, dp, dc, st);
+    hipLaunchKernelGGL((col_pass_kernel<R, N, E, W, G>), g, b, l, s, i, r, dpsi, dn, (const C*)tw, m, dp, dc, st);
 }
 template <typename R, int N, int E, int ROWS, bool G>
 void row_launch(dim3 g, dim3 b, size_t l, hipStream_t s, hipEvent_t e0, hipEvent_t e1, const InstDev* i, RunDev* r,
-                const void* tw, int dur, int first, int colw, int st) {
+                void* dpsi, int dn, const void* tw, int dur, int first, int colw, int st) {
   typedef typename Cx<R>::T C;
   if (e0)
-    hipExtLaunchKernelGGL((row_pass_kernel<R, N, E, ROWS, G>), g, b, (uint32_t)l, s, e0, e1, 0, i, r, (const C*)tw,
-                          dur, first, colw, st);
+    hipExtLaunchKernelGGL((row_pass_kernel<R, N, E, ROWS, G>), g, b, (uint32_t)l, s, e0, e1, 0, i, r, dpsi, dn,
+                          (const C*)tw, dur, first, colw, st);
   else
-    hipLaunchKernelGGL((row_pass_kernel<R, N, E, ROWS, G>), g, b, l, s, i, r, (const C*)tw, dur, first, colw, st);
+    hipLaunchKernelGGL((row_pass_kernel<R, N, E, ROWS, G>), g, b, l, s, i, r, dpsi, dn, (const C*)tw, dur, first, colw,
+                       st);
 }
 template <typename R, int N, int E, int ROWS>
 void frow_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const void* in, void* out, int inv, int shift, double scale,
