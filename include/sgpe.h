@@ -131,7 +131,9 @@ void sgpe_plan_destroy(sgpe_plan* plan);
 int sgpe_plan_set_slot(sgpe_plan* plan, int slot, const sgpe_inst_desc* desc, void* psik);
 
 /* Rebind a slot to a (new) normalized k-space state psik and start a fresh
- * run; physics tables are kept. Synchronizes the stream. */
+ * run; physics tables are kept. Synchronizes the stream. On a one-slot plan
+ * the pass kernels take the state address as an argument, so its captured
+ * step graphs are dropped and captured again on the next graph call. */
 int sgpe_plan_set_state(sgpe_plan* plan, int slot, void* psik);
 
 /* Start a run for one (set) slot and activate it: device buffer pops (2 doubles
