@@ -130,6 +130,33 @@ def test_graph_and_eager_are_bit_identical(gpu):
     np.testing.assert_array_equal(outs[0], outs[1])
 
 
+def test_rebinding_after_graph_capture(gpu):
+    """A one-slot plan's captured step graphs hold its state address; after
+    sgpe_plan_set_state the next graph call must advance the new state and
+    leave the old buffer alone (bit-identical to a plan bound to the new
+    state from the start)."""
+    from spinor_gpe.engine.plan import InstanceSpec, Plan
+    ps, time, dt = CASES['raman_lab'](name='rebind')
+    spec = InstanceSpec.from_spin(ps, time, dt)
+    plan = Plan(gpu, 'c128', spec.ny, spec.nx, 1)
+    first = plan.new_state(ps.psik)
+    plan.set_slot(0, spec, first)
+    plan.graph_steps(8, 2)
+    plan.sync()
+    first_after = first.clone()
+    second = plan.new_state(ps.psik)
+    plan.set_state(0, second)
+    plan.graph_steps(8, 2)
+    plan.sync()
+    np.testing.assert_array_equal(first.cpu().numpy(), first_after.cpu().numpy())
+    ref = Plan(gpu, 'c128', spec.ny, spec.nx, 1)
+    st = ref.new_state(ps.psik)
+    ref.set_slot(0, spec, st)
+    ref.graph_steps(8, 2)
+    ref.sync()
+    np.testing.assert_array_equal(second.cpu().numpy(), st.cpu().numpy())
+
+
 def test_convergence_stop(gpu):
     """conv_tol stops the run once |dE/E| per step <= tol (checked per block)."""
     ps, time, dt = CASES['ground'](name='conv')
