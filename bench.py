@@ -106,7 +106,8 @@ def c64_vs_c128(r64, r128, dv_k):
     """Errors of the complex64 run against the complex128 run of the same
     steps from the same state: state fidelity |<a|b>|^2 / (<a|a><b|b>),
     relative GP energy difference, and population differences (of N_at)."""
-    a, b = r64['state'], r128['state']
+    a = np.asarray(r64['state'], dtype=np.complex128)  # sums in float64 (a complex64 norm is off by ~1e-7)
+    b = np.asarray(r128['state'], dtype=np.complex128)
     fid = abs(np.vdot(b, a)) ** 2 / (np.vdot(a, a).real * np.vdot(b, b).real)
     pops_a = [float(np.sum(np.abs(c) ** 2)) * dv_k for c in a]
     pops_b = [float(np.sum(np.abs(c) ** 2)) * dv_k for c in b]
@@ -129,20 +130,27 @@ def sweep_leg(device, check_every, fp64_sample):
     spins = [wl.raman_spinor(f'sweep{i}', d) for i, d in enumerate(draws)]
     conv, cap, dt = 1e-8, 8000, 0.02
 
-    def solve(prec, items):
+    def solve(prec, items, keep=False):
         sw = Sweep(device, precision=prec, conv_tol=conv, check_every=check_every, max_steps=cap)
         for ps in items:
             sw.add_spin(ps, dt)
-        res, wall = sw.run(keep_states=False)
+        res, wall = sw.run(keep_states=keep)  # states are copied out after the timed loop
         return res, wall, sw.trace
 
     log("sweep: 96 instances, complex64 batched")
-    res32, wall32, trace32 = solve('c64', spins)
+    res32, wall32, trace32 = solve('c64', spins, keep=True)
     log(f"sweep: c64 batched {wall32:.2f} s; complex128 batched (accuracy reference)")
-    res64, wall64, _ = solve('c128', spins)
+    res64, wall64, _ = solve('c128', spins, keep=True)
     rel = np.array([abs(a.energy['E_tot'] - b.energy['E_tot']) / abs(b.energy['E_tot'])
                     for a, b in zip(res32, res64)])
     ok = np.array([a.converged and r <= 1e-6 for a, r in zip(res32, rel)])
+    errs = [c64_vs_c128({'state': a.psik, 'energy': [a.energy['E_tot']], 'steps_total': a.steps},
+                        {'state': b.psik, 'energy': [b.energy['E_tot']], 'steps_total': b.steps},
+                        ps.space['dv_k']) for a, b, ps in zip(res32, res64, spins)]
+    fid = np.array([e['state_fidelity'] for e in errs])
+    pops = np.array([e['pops_err_of_atoms'] for e in errs])
+    for r in res32 + res64:
+        r.psik = None
     single = {}
     for grid in sorted({d.grid for d in draws}):
         idx = [i for i, d in enumerate(draws) if d.grid == grid][:fp64_sample]
@@ -167,6 +175,11 @@ def sweep_leg(device, check_every, fp64_sample):
         'steps_mean': float(steps.mean()), 'steps_max': int(steps.max()),
         'active_trace': [[round(t, 3), n, a] for t, n, a in trace32],
         'energy_rel_err_vs_c128': {'max': float(rel.max()), 'median': float(np.median(rel))},
+        'state_fidelity_vs_c128': {'min_counted': float(fid[ok].min()), 'median': float(np.median(fid)),
+                                   'min': float(fid.min()),
+                                   'counted_above_0.999': int(np.sum(fid[ok] >= 0.999))},
+        'pops_err_of_atoms_vs_c128': {'max_counted': float(pops[ok].max()), 'median': float(np.median(pops)),
+                                      'max': float(pops.max())},
         'c128_batched': {'wall_s': round(wall64, 3), 'converged': int(sum(r.converged for r in res64)),
                          'solves_per_s': round(len(draws) / wall64, 3)},
         'fp64_single': {'sampled_per_grid': fp64_sample,

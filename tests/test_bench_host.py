@@ -78,6 +78,12 @@ def test_c64_vs_c128_errors():
     same = bench.c64_vs_c128({'state': b * np.exp(0.3j), 'energy': np.array([2.0]), 'steps_total': 10},
                              r128, 0.5)
     assert same['state_fidelity'] == pytest.approx(1.0, abs=1e-14)  # a global phase is no error
+    # a complex64 state is compared in float64 sums: its own fidelity is 1 to rounding of float64
+    big = rng.normal(size=(2, 256, 256)) + 1j * rng.normal(size=(2, 256, 256))
+    f32 = big.astype(np.complex64)
+    own = bench.c64_vs_c128({'state': f32, 'energy': np.array([2.0]), 'steps_total': 1},
+                            {'state': f32.astype(np.complex128), 'energy': np.array([2.0]), 'steps_total': 1}, 1.0)
+    assert abs(own['state_fidelity'] - 1.0) < 1e-12
     assert same['energy_rel_err'] == 0.0 and same['pops_err_of_atoms'] == pytest.approx(0.0, abs=1e-15)
     swapped = np.stack([b[0], np.zeros_like(b[1])])
     other = {'state': np.stack([np.zeros_like(b[0]), b[1]]), 'energy': np.array([2.2]), 'steps_total': 10}
