@@ -225,7 +225,40 @@ def serve_leg(device, n_requests, load):
         'slots': {str(k): v for k, v in srv.slots.items()}, 'block_steps': srv.block,
         'capacity_solves_per_s': round(capacity, 3), 'arrival_rate_per_s': round(rate, 3),
         'load': load, 'burst': serve.summarize(burst, makespan)})
+    res['ground_state_errors_vs_c128'] = serve_accuracy(device, n_requests)
     return res
+
+
+def serve_accuracy(device, n_requests):
+    """Errors of the served complex64 ground states against complex128 solves
+    of the same draws (same step, stop test and cap): the requests are served
+    once more, untimed, keeping their states, and the ground-state ones are
+    solved as a complex128 sweep."""
+    from spinor_gpe.engine import serve
+    from spinor_gpe.engine import workloads as wl
+    from spinor_gpe.engine.sweep import Sweep
+    log("serve: accuracy of the served ground states against complex128 solves")
+    srv = serve.Server(device, 'c64', keep_states=True)
+    reqs = serve.request_stream(n_requests, None, seed=3)
+    draws = {r.rid: r.draw for r in reqs}
+    ground = [o for o in srv.run(reqs) if o.kind == 'ground']
+    sw = Sweep(device, precision='c128', conv_tol=srv.conv_tol, check_every=srv.block, max_steps=srv.max_steps)
+    spins = [wl.raman_spinor(f'serve_ref{o.rid}', draws[o.rid]) for o in ground]
+    for ps in spins:
+        sw.add_spin(ps, srv.dt_imag)
+    ref, _ = sw.run(keep_states=True)
+    errs = [c64_vs_c128({'state': o.psik, 'energy': [o.energy['E_tot']], 'steps_total': o.steps_imag},
+                        {'state': r.psik, 'energy': [r.energy['E_tot']], 'steps_total': r.steps},
+                        ps.space['dv_k']) for o, r, ps in zip(ground, ref, spins)]
+    both = [o.converged and r.converged for o, r in zip(ground, ref)]
+    e = np.array([x['energy_rel_err'] for x in errs])
+    f = np.array([x['state_fidelity'] for x in errs])
+    p = np.array([x['pops_err_of_atoms'] for x in errs])
+    return {'ground_requests': len(ground), 'both_converged': int(sum(both)),
+            'energy_rel_err': {'median': float(np.median(e)), 'max': float(e.max()),
+                               'within_1e-6': int(np.sum(e <= 1e-6))},
+            'state_fidelity': {'median': float(np.median(f)), 'min': float(f.min())},
+            'pops_err_of_atoms': {'median': float(np.median(p)), 'max': float(p.max())}}
 
 
 def measured_traffic(kernel):
