@@ -325,7 +325,7 @@ def roofline(res, prec, n):
     real = 'double' if prec == 'c128' else 'float'
     kname = (f"row_pass_kernel<{real}, {n}, {row_e}, {row_rows}>" if dom == 'row_pass'
              else f"col_pass_kernel<{real}, {n}, {col_e}, {col_w}>")
-    if res['streaming'][0 if dom == 'row_pass' else 1] & 1:
+    if res['streaming'][0 if dom == 'row_pass' else 1]:
         kname += ' streaming'
     traffic = measured_traffic(kname)
     return {'bound': 'hbm', 'kernel': dom, 'achieved': round(achieved, 1),

@@ -167,10 +167,7 @@ int sgpe_plan_get_tuning(const sgpe_plan* plan, int* out4);
 
 /* Cache policy of the two passes' state reads and writes: 1 = non-temporal
  * (streaming) accesses, 0 = normal. Default: streaming for single-slot plans,
- * normal for batched ones. col_stream bit 1 (value 2) makes the column pass
- * read its y phase tables from global memory instead of staging them in LDS
- * (more workgroups per CU on some batched shapes). No reference counterpart
- * (launch configuration). */
+ * normal for batched ones. No reference counterpart (launch configuration). */
 int sgpe_plan_set_streaming(sgpe_plan* plan, int row_stream, int col_stream);
 int sgpe_plan_get_streaming(const sgpe_plan* plan, int* out2);
 

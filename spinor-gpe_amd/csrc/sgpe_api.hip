@@ -202,8 +202,8 @@ static const int kColTables = 2;
 
 // Dynamic LDS of a column pass: exchange buffer, tables, and the separate
 // slot for its end-of-pass sums when it fits (col_sum_separate).
-static size_t col_lds_bytes(int w, int n, size_t cs, bool ky_global = false) {
-  const size_t base = lds_bytes((size_t)w * n * cs, n, cs, ky_global ? 0 : kColTables);
+static size_t col_lds_bytes(int w, int n, size_t cs) {
+  const size_t base = lds_bytes((size_t)w * n * cs, n, cs, kColTables);
   return base + (col_sum_separate(base) ? kColSumBytes : 0);
 }
 
@@ -306,7 +306,7 @@ static void* direct_state(const sgpe_plan* p) {
 
 static void launch_col(sgpe_plan* p, hipStream_t s, int mode, int dprev, int dcur, hipEvent_t e0 = nullptr,
                        hipEvent_t e1 = nullptr) {
-  const size_t lds = col_lds_bytes(p->col_w, p->ny, csize(p->prec), (p->col_stream & 2) != 0);
+  const size_t lds = col_lds_bytes(p->col_w, p->ny, csize(p->prec));
   dim3 grid(p->nx / p->col_w, 2, p->nslots), block(p->col_w * (p->ny / p->col_e));
   ((ColFn)(p->dense_kin ? p->col->launch_g : p->col->launch))(grid, block, lds, s, e0, e1, p->d_inst, p->d_run,
                                                               direct_state(p), p->nx, p->tw_y, mode, dprev, dcur,
@@ -950,7 +950,7 @@ int sgpe_plan_set_streaming(sgpe_plan* p, int row_stream, int col_stream) {
   return guarded([&]() -> int {
     if (!p) return fail(SGPE_E_ARG, "plan is NULL");
     p->row_stream = row_stream ? 1 : 0;
-    p->col_stream = col_stream & 3;
+    p->col_stream = col_stream ? 1 : 0;
     drop_graphs(p);
     return SGPE_OK;
   });

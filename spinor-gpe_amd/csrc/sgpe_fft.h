@@ -410,10 +410,4 @@ __device__ __forceinline__ void block_sum_thread0(double (&v)[NV], double* scrat
 constexpr size_t kColSumBytes = 256;
 constexpr bool col_sum_separate(size_t base_bytes) { return base_bytes + kColSumBytes <= kLdsBytes; }
 
-// Whether a column pass's two N-entry y phase tables fit in LDS next to its
-// exchange data and twiddles (same predicate on host and device).
-constexpr bool col_ky_fits(size_t data_bytes, int n, size_t elem_bytes) {
-  return tw_in_lds(data_bytes, n, elem_bytes) && data_bytes + 3 * (size_t)n * elem_bytes <= kLdsBytes;
-}
-
 }  // namespace sgpe

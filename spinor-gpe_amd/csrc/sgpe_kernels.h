@@ -233,17 +233,12 @@ __global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __re
   typedef LdsView<W, 1 << 30, 0> V;
   constexpr int T = N / E;
   // LDS: [W*N exchange][N twiddles][N ky_prev][N ky_cur][sums] (tables and
-  // a separate sum slot when they fit). `stream` bit 0: non-temporal state
-  // accesses; bit 1: the y tables are read from global memory even where
-  // they fit in LDS (a per-plan choice: staging them shortens a single
-  // instance's pass, leaving them out keeps one more workgroup per CU on
-  // some batched shapes).
+  // a separate sum slot when they fit)
   constexpr size_t kData = (size_t)W * N * sizeof(C);
   constexpr bool kTw = tw_in_lds(kData, N, sizeof(C));
-  constexpr bool kKy = col_ky_fits(kData, N, sizeof(C));
-  const bool nt = stream & 1;
-  const bool ky_lds = kKy && !(stream & 2);
-  const size_t base = kData + (kTw ? (size_t)N * sizeof(C) : 0) + (ky_lds ? 2 * (size_t)N * sizeof(C) : 0);
+  constexpr bool kKy = kTw && kData + 3 * (size_t)N * sizeof(C) <= kLdsBytes;
+  constexpr size_t kBase = kData + (kTw ? (size_t)N * sizeof(C) : 0) + (kKy ? 2 * (size_t)N * sizeof(C) : 0);
+  constexpr bool kSumSep = col_sum_separate(kBase);
   extern __shared__ double smem[];
   const int inst = blockIdx.z;
   const InstDev& P = insts[inst];
@@ -269,7 +264,7 @@ __global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __re
   // then the kinetic tables and twiddles (staged into LDS).
   constexpr int kBlock = W * T;
   C a[E];
-  if (nt) {
+  if (stream) {
 #pragma unroll
     for (int j = 0; j < E; ++j) a[j] = ld_stream(col + (size_t)(t + j * T) * nx);
   } else {
@@ -298,8 +293,8 @@ __global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __re
     if (!runs[inst].active) return;
   }
   const bool kspace = runs[inst].form == FORM_KSPACE;
-  const C* kyp = ky_lds ? sp.store(sm + W * N + N) : sp.src;
-  const C* kyc = ky_lds ? sc_.store(sm + W * N + 2 * N) : sc_.src;
+  const C* kyp = sp.store(sm + W * N + N);
+  const C* kyc = sc_.store(sm + W * N + 2 * N);
   const C* tw = stw.store(sm + W * N);
 
   if (!kspace)
@@ -320,7 +315,7 @@ __global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __re
       col_kinetic<R, E, T, true, false>(a, t, nx, jx, td, hp, hc, kxp, kxc, kyp, kyc, kspace, mid, s_acc, n_acc);
   }
   if (mode == COL_MID) fft_run<N, E, true, V>(a, t, tw, lds);
-  if (nt) {
+  if (stream) {
 #pragma unroll
     for (int j = 0; j < E; ++j) st_stream(col + (size_t)(t + j * T) * nx, a[j]);
   } else {
@@ -329,14 +324,14 @@ __global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __re
   }
 
   double v[2] = {s_acc, n_acc};
-  if (col_sum_separate(base))
-    block_sum_thread0<2>(v, reinterpret_cast<double*>(reinterpret_cast<char*>(smem) + base));
+  if constexpr (kSumSep)
+    block_sum_thread0<2>(v, reinterpret_cast<double*>(reinterpret_cast<char*>(smem) + kBase));
   else
     block_sum<2>(v, smem);
   if (threadIdx.x == 0) {
-    const int ntiles = nx / W;
-    P.part[PART_S * 2 * ntiles + comp * ntiles + tile] = v[0];
-    P.part[PART_NRM * 2 * ntiles + comp * ntiles + tile] = v[1];
+    const int nt = nx / W;
+    P.part[PART_S * 2 * nt + comp * nt + tile] = v[0];
+    P.part[PART_NRM * 2 * nt + comp * nt + tile] = v[1];
   }
 }
 

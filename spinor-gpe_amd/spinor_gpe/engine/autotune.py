@@ -6,16 +6,12 @@ columns (column pass) per workgroup; each pass can also stream its state
 accesses past the caches (non-temporal) or not. Which choice is fastest
 depends on the grid, the precision and how many instances share a launch
 (small single instances are latency-bound and want many small waves and
-streaming; large or batched ones want fewer, fatter threads). The column
-pass can also read its y phase tables from global memory instead of staging
-them in LDS (bit 1 of its flag): staging shortens a single instance's pass,
-leaving them out keeps a workgroup per CU more on some batched shapes. The
-tuner times every compiled shape in every mode with
-``sgpe_profile_full_steps`` on a scratch plan (the caller's states are never
-touched), picks the fastest row and column choices independently and caches
-them in a JSON file keyed by precision, grid and batch-size bucket as
-``[row E, rows, column E, columns, row streaming, column flags]`` (column
-flags: bit 0 streaming, bit 1 y tables from global memory).
+streaming; large or batched ones want fewer, fatter threads). The tuner times
+every compiled shape in both cache modes with ``sgpe_profile_full_steps`` on
+a scratch plan (the caller's states are never touched), picks the fastest
+row and column choices independently and caches them in a JSON file keyed by
+precision, grid and batch-size bucket as
+``[row E, rows, column E, columns, row streaming, column streaming]``.
 
 Usage: ``python -m spinor_gpe.engine.autotune --grids 128 256 512 1024 2048``
 writes the cache (``$SGPE_TUNE_FILE``, default
@@ -148,7 +144,7 @@ def tune(device, precision, ny, nx, n_slots=1, steps=30, reps=3, verbose=False):
     best = list(base) + list(base_st)
     for pass_, name in ((0, 'row'), (1, 'col')):
         best_t = float('inf')
-        for st in ((0, 1) if pass_ == 0 else (0, 1, 2, 3)):
+        for st in (0, 1):
             for e, p in shapes(pass_, precision, nx if pass_ == 0 else ny):
                 try:
                     if pass_ == 0:
@@ -162,10 +158,9 @@ def tune(device, precision, ny, nx, n_slots=1, steps=30, reps=3, verbose=False):
                 plan.profile_steps(2)
                 idx = 1 if pass_ == 0 else 0  # profile_steps -> (col ms, row ms, ...)
                 t = min(plan.profile_steps(steps)[idx] for _ in range(reps)) / (3 * steps) * 1e3
-                timings[name][f"{e}x{p}" + ("s" if st & 1 else "") + ("g" if st & 2 else "")] = round(t, 3)
+                timings[name][f"{e}x{p}" + ("s" if st else "")] = round(t, 3)
                 if verbose:
-                    print(f"  {name} E={e} {p}{' streaming' if st & 1 else ''}{' y-tables global' if st & 2 else ''}: "
-                          f"{t:.2f} us", flush=True)
+                    print(f"  {name} E={e} {p}{' streaming' if st else ''}: {t:.2f} us", flush=True)
                 if t < best_t:
                     best_t = t
                     if pass_ == 0:

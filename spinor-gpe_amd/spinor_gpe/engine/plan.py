@@ -196,9 +196,8 @@ class Plan:
         return tuple(out)
 
     def set_streaming(self, row_stream, col_stream):
-        """Non-temporal (1) or normal (0) state accesses of each pass; the
-        column flag's bit 1 (2) reads its y phase tables from global memory."""
-        self._call('sgpe_plan_set_streaming', int(bool(row_stream)), int(col_stream) & 3)
+        """Non-temporal (1) or normal (0) state accesses of each pass."""
+        self._call('sgpe_plan_set_streaming', int(bool(row_stream)), int(bool(col_stream)))
 
     def streaming(self):
         out = (ctypes.c_int * 2)()

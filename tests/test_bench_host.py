@@ -23,7 +23,7 @@ def _packaged_table_only(monkeypatch, tmp_path):
 def _instance(kind, prec, n, shape):
     real = 'double' if prec == 'c128' else 'float'
     e, p, stream = (shape[0], shape[1], shape[4]) if kind == 'row' else (shape[2], shape[3], shape[5])
-    return f"{kind}_pass_kernel<{real}, {n}, {e}, {p}>" + (' streaming' if stream & 1 else '')
+    return f"{kind}_pass_kernel<{real}, {n}, {e}, {p}>" + (' streaming' if stream else '')
 
 
 @pytest.mark.parametrize('prec', ['c128', 'c64'])

@@ -85,40 +85,3 @@ def test_shape_matches_oracle(gpu, n, prec, pass_, shape, general):
     got = st.cpu().numpy()
     err = np.linalg.norm(got - want) / np.linalg.norm(want)
     assert err < (1e-12 if prec == 'c128' else 2e-5), err
-
-
-def table_cases():
-    out = []
-    for n in (256, 1024):
-        for prec in ('c128', 'c64'):
-            for shape in autotune.shapes(1, prec, n):
-                out.append((n, prec, shape, False))
-                if n in GENERAL_GRIDS:
-                    out.append((n, prec, shape, True))
-    return out
-
-
-@pytest.mark.parametrize('n,prec,shape,general', table_cases())
-def test_column_tables_from_global_match_oracle(gpu, n, prec, shape, general):
-    """Column flag bit 1: the y phase tables read from global memory instead
-    of LDS (a smaller LDS allocation; the end-of-pass sums move with it)."""
-    print(f"[shape] n={n} {prec} col shape={shape} y tables global, general={general}", flush=True)
-    ps, time, dt, want = reference(n, general)
-    spec = InstanceSpec.from_spin(ps, time, dt)
-    plan = Plan(gpu, prec, n, n, 1)
-    cur = plan.tuning()
-    try:
-        plan.tune(cur[0], cur[1], shape[0], shape[1])
-    except nat.SgpeError as err:
-        assert 'LDS' in str(err) or 'divide' in str(err)
-        pytest.skip(f"shape not usable here: {err}")
-    row_st, col_st = plan.streaming()
-    plan.set_streaming(row_st, col_st | 2)
-    assert plan.streaming()[1] == col_st | 2
-    st = plan.new_state(ps.psik)
-    plan.set_slot(0, spec, st)
-    plan.full_steps(steps(n))
-    plan.sync()
-    got = st.cpu().numpy()
-    err = np.linalg.norm(got - want) / np.linalg.norm(want)
-    assert err < (1e-12 if prec == 'c128' else 2e-5), err
