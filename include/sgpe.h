@@ -83,6 +83,11 @@ int sgpe_abi_version(void);
 /* Message of the last failed call on this thread ("" if none). */
 const char* sgpe_last_error(void);
 
+/* Test hook (no reference counterpart): the next n device allocations of the
+ * library fail as if the device were out of memory, so that tests can reach
+ * the SGPE_E_NOMEM paths of the entry points. 0 restores normal behaviour. */
+int sgpe_debug_fail_allocs(int n);
+
 /* ---------------------------------------------------------------- context */
 
 /* Bind a device and a stream (hipStream_t, NULL = default stream). Replaces

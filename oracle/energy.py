@@ -82,16 +82,28 @@ def phase_masked(psi_comp, dens, unwrap):
     return ang
 
 
+PLANTS = ('half_int', 'area', 'cpl_theta', 'recoil_axis1')
+
+
 def eng_expect(psik, dr, pot_eng_spin, coupling, g_sc, kl_recoil, is_coupling,
-               unwrap=unwrap_phase_py):
-    """tensor_propagator.py:293-324 on NumPy arrays."""
+               unwrap=unwrap_phase_py, plant=None, theta=None):
+    """tensor_propagator.py:293-324 on NumPy arrays.
+
+    ``plant`` (tests only) introduces one named mistake, so that the tests
+    can show their closed forms catch it: 'half_int' (a 1/2 on the
+    interaction terms), 'area' (the total times dv_r), 'cpl_theta' (the
+    coupling term with its phase ``theta``), 'recoil_axis1' (the recoil term
+    on the axis-1 phase derivative).
+    """
+    if plant is not None and plant not in PLANTS:
+        raise ValueError(plant)
     dr = np.asarray(dr)
     psi = ifft_2d_np(psik, dr)
     dens = [np.abs(p) ** 2 for p in psi]
     dens_sqrt = [np.sqrt(d) for d in dens]
     phase = [phase_masked(p, d, unwrap) for p, d in zip(psi, dens)]
     grads = [np.gradient(p, *dr) for p in phase]
-    phase_gradx = [g[0] for g in grads]
+    phase_gradx = [g[1] if plant == 'recoil_axis1' else g[0] for g in grads]
     phase_gradsq = [g[0] ** 2 + g[1] ** 2 for g in grads]
     root_gradsq = []
     for d in dens_sqrt:
@@ -102,8 +114,13 @@ def eng_expect(psik, dr, pot_eng_spin, coupling, g_sc, kl_recoil, is_coupling,
            + sum([d * p for d, p in zip(dens, phase_gradx)]) * 2 * kl_recoil * is_coupling) / 2
     pot = sum([d * v for d, v in zip(dens, pot_eng_spin)])
     int_e = g_sc['uu'] * dens[0] ** 2 + g_sc['dd'] * dens[1] ** 2 + g_sc['ud'] * dens[0] * dens[1]
-    coupl_e = (np.conj(psi[0]) * psi[1] + np.conj(psi[1]) * psi[0]) * np.asarray(coupling) / 2
+    if plant == 'half_int':
+        int_e = 0.5 * int_e
+    eth = np.exp(-1j * np.asarray(theta)) if plant == 'cpl_theta' else 1.0
+    coupl_e = (np.conj(psi[0]) * psi[1] * eth + np.conj(psi[1]) * psi[0] * np.conj(eth)) * np.asarray(coupling) / 2
     total = np.real((kin + pot + int_e + coupl_e).sum())
+    if plant == 'area':
+        total = total * dr[0] * dr[1]
     return [total, np.real(kin).sum(), np.real(pot).sum(), np.real(int_e).sum()]
 
 

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -94,6 +95,21 @@ static const KEntry* find_entry(int kind, int prec, int n, int e, int p) {
 
 static const size_t kMaxLds = kLdsBytes;
 
+// Device allocations of the library go through dev_alloc, so that a test can
+// make the next n of them fail (sgpe_debug_fail_allocs) and exercise the
+// failure paths of the entry points.
+static std::atomic<int> g_fail_allocs{0};
+static hipError_t dev_alloc(void** ptr, size_t bytes) {
+  int n = g_fail_allocs.load();
+  while (n > 0) {
+    if (g_fail_allocs.compare_exchange_weak(n, n - 1)) {
+      *ptr = nullptr;
+      return hipErrorOutOfMemory;
+    }
+  }
+  return hipMalloc(ptr, bytes);
+}
+
 static size_t csize(int prec) { return prec == SGPE_C128 ? 16 : 8; }
 
 // Default shapes per length (before any tuning).
@@ -179,7 +195,7 @@ static int ctx_twiddles(sgpe_ctx* ctx, int prec, int n, int e, const void** out)
   }
   void* d = nullptr;
   HIP_TRY(hipSetDevice(ctx->device));
-  if (hipMalloc(&d, host.size()) != hipSuccess) return fail(SGPE_E_NOMEM, "twiddle table allocation failed");
+  if (dev_alloc(&d, host.size()) != hipSuccess) return fail(SGPE_E_NOMEM, "twiddle table allocation failed");
   HIP_TRY(hipMemcpy(d, host.data(), host.size(), hipMemcpyHostToDevice));
   ctx->tw[key] = d;
   *out = d;
@@ -218,7 +234,7 @@ static int ctx_reduction(sgpe_ctx* ctx, size_t count) {
   if (ctx->red_count >= count) return SGPE_OK;
   if (ctx->red) HIP_TRY(hipFree(ctx->red));
   ctx->red = nullptr;
-  if (hipMalloc(&ctx->red, count * sizeof(double)) != hipSuccess) return fail(SGPE_E_NOMEM, "scratch allocation failed");
+  if (dev_alloc((void**)&ctx->red, count * sizeof(double)) != hipSuccess) return fail(SGPE_E_NOMEM, "scratch allocation failed");
   ctx->red_count = count;
   return SGPE_OK;
 }
@@ -248,6 +264,7 @@ struct sgpe_plan {
   std::vector<InstDev> h_inst;
   std::vector<Slot> slots;
   int last_dur = DUR_OUT;  // duration of the last sub-step issued (closing K pending)
+  bool pending = false;    // a sub-step was issued since the last sync (active states are hybrid)
   std::map<int, hipGraphExec_t> graphs;
   std::map<int, hipGraph_t> graph_defs;
   void* scratch = nullptr;  // [nslots][2][ny][nx] for energy evaluation
@@ -326,6 +343,7 @@ static void issue_sub_step(sgpe_plan* p, hipStream_t s, int which, int first) {
   launch_col(p, s, COL_MID, p->last_dur, which);
   launch_row(p, s, which, first);
   p->last_dur = which;
+  p->pending = true;
 }
 
 static void issue_full_step(sgpe_plan* p, hipStream_t s) {
@@ -346,6 +364,7 @@ static void issue_sync(sgpe_plan* p, hipStream_t s) {
   hipLaunchKernelGGL(set_form_kernel, dim3((p->nslots + 63) / 64), dim3(64), 0, s, p->d_run, p->nslots,
                      (int)FORM_KSPACE);
   p->last_dur = DUR_OUT;
+  p->pending = false;
 }
 
 struct sgpe_group {
@@ -376,7 +395,7 @@ static int plan_energy_prepare(sgpe_plan* p) {
   if ((rc = set_lds_attr(p->erow->kernel, lds_bytes((size_t)p->e_row_rows * 2 * p->nx * cs, p->nx, cs)))) return rc;
   if ((rc = ctx_twiddles(p->ctx, p->prec, p->ny, p->e_col_e, &p->etw_y))) return rc;
   if ((rc = ctx_twiddles(p->ctx, p->prec, p->nx, p->e_row_e, &p->etw_x))) return rc;
-  if (hipMalloc(&p->scratch, per * p->nslots) != hipSuccess) {
+  if (dev_alloc(&p->scratch, per * p->nslots) != hipSuccess) {
     p->scratch = nullptr;
     return fail(SGPE_E_NOMEM, "energy work buffer allocation failed");
   }
@@ -570,15 +589,20 @@ static int plan_load_slot(sgpe_plan* p, int slot, const sgpe_inst_desc* d, void*
       put_cplx(st, o_eth, prec, i, 1.0, 0.0);
   }
 
+  // A slot whose tables grow gets a new buffer; the old one is released
+  // only once the new tables are in place, so a failed allocation leaves
+  // the slot's previous physics, state and run untouched.
   Slot& sl = p->slots[slot];
-  if (sl.bytes < st.bytes.size()) {
-    if (sl.buf) HIP_TRY(hipFree(sl.buf));
-    sl.buf = nullptr;
-    sl.bytes = 0;
-    if (hipMalloc(&sl.buf, st.bytes.size()) != hipSuccess) return fail(SGPE_E_NOMEM, "slot table allocation failed");
-    sl.bytes = st.bytes.size();
-  }
-  char* base = (char*)sl.buf;
+  void* fresh = nullptr;
+  if (sl.bytes < st.bytes.size() && dev_alloc(&fresh, st.bytes.size()) != hipSuccess)
+    return fail(SGPE_E_NOMEM, "slot table allocation failed");
+  struct FreshGuard {
+    void*& ptr;
+    ~FreshGuard() {
+      if (ptr) (void)hipFree(ptr);
+    }
+  } guard{fresh};
+  char* base = (char*)(fresh ? fresh : sl.buf);
   auto P = [&](size_t off) -> char* { return off ? base + off : nullptr; };
   for (int s = 0; s < 2; ++s) {
     h.t_dense[s] = (const double*)P(o_td[s]);
@@ -602,12 +626,18 @@ static int plan_load_slot(sgpe_plan* p, int slot, const sgpe_inst_desc* d, void*
   h.psi = psik;
 
   hipStream_t s = p->ctx->stream;
-  HIP_TRY(hipMemcpyAsync(sl.buf, st.bytes.data(), st.bytes.size(), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(base, st.bytes.data(), st.bytes.size(), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(p->d_inst + slot, &h, sizeof(h), hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(config_run_kernel, dim3(1), dim3(64), 0, s, p->d_run, slot, (double*)nullptr, 0, 1, 1,
                      (int)FORM_KSPACE, 0.0);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipStreamSynchronize(s));  // staging buffer goes out of scope
+  HIP_TRY(hipStreamSynchronize(s));  // staging buffer goes out of scope; old tables unused from here on
+  if (fresh) {
+    if (sl.buf) (void)hipFree(sl.buf);
+    sl.buf = fresh;
+    sl.bytes = st.bytes.size();
+    fresh = nullptr;  // kept by the slot
+  }
   p->h_inst[slot] = h;
   sl.loaded = true;
   // Kernel variants: the dense-operator ones once any loaded slot needs them.
@@ -641,6 +671,12 @@ static int plan_ready(const sgpe_plan* p) {
 extern "C" {
 
 int sgpe_abi_version(void) { return SGPE_ABI_VERSION; }
+
+int sgpe_debug_fail_allocs(int n) {
+  if (n < 0) return fail(SGPE_E_ARG, "n < 0");
+  g_fail_allocs.store(n);
+  return SGPE_OK;
+}
 
 const char* sgpe_last_error(void) { return g_err.c_str(); }
 
@@ -821,9 +857,9 @@ int sgpe_plan_create(sgpe_ctx* ctx, int precision, int ny, int nx, int n_slots, 
     default_row(precision, nx, &re, &rr);
     default_col(precision, ny, &ce, &cw);
     int rc = plan_select(p, re, rr, ce, cw);
-    if (!rc && hipMalloc((void**)&p->d_inst, n_slots * sizeof(InstDev)) != hipSuccess)
+    if (!rc && dev_alloc((void**)&p->d_inst, n_slots * sizeof(InstDev)) != hipSuccess)
       rc = fail(SGPE_E_NOMEM, "slot table allocation failed");
-    if (!rc && hipMalloc((void**)&p->d_run, n_slots * sizeof(RunDev)) != hipSuccess)
+    if (!rc && dev_alloc((void**)&p->d_run, n_slots * sizeof(RunDev)) != hipSuccess)
       rc = fail(SGPE_E_NOMEM, "run state allocation failed");
     if (!rc && hipMemset(p->d_run, 0, n_slots * sizeof(RunDev)) != hipSuccess)
       rc = fail(SGPE_E_HIP, "run state initialization failed");
@@ -867,7 +903,7 @@ int sgpe_plan_set_state(sgpe_plan* p, int slot, void* psik) {
     HIP_TRY(hipSetDevice(p->ctx->device));
     InstDev& h = p->h_inst[slot];
     h.psi = psik;
-    drop_graphs(p);  // captured single-slot passes hold the state address
+    if (p->nslots == 1) drop_graphs(p);  // captured single-slot passes hold the state address
     HIP_TRY(hipMemcpyAsync(&p->d_inst[slot].psi, &h.psi, sizeof(void*), hipMemcpyHostToDevice, p->ctx->stream));
     hipLaunchKernelGGL(config_run_kernel, dim3(1), dim3(64), 0, p->ctx->stream, p->d_run, slot, (double*)nullptr, 0, 1, 1,
                        (int)FORM_KSPACE, 0.0);
@@ -912,6 +948,10 @@ int sgpe_plan_set_active(sgpe_plan* p, int slot, int active) {
     if (!p || slot < 0 || slot >= p->nslots) return fail(SGPE_E_ARG, "bad arguments");
     if (active && !p->slots[slot].loaded) return fail(SGPE_E_ARG, "slot has not been set");
     HIP_TRY(hipSetDevice(p->ctx->device));
+    // A disabled slot keeps its state as it is; finish the pending sub-step
+    // first so that it is frozen as normalized k-space (on re-enabling, the
+    // next step then starts from k-space, whatever sub-steps ran meanwhile).
+    if (!active && p->pending) issue_sync(p, p->ctx->stream);
     hipLaunchKernelGGL(set_active_kernel, dim3(1), dim3(64), 0, p->ctx->stream, p->d_run, slot, active ? 1 : 0);
     HIP_TRY(hipGetLastError());
     return SGPE_OK;
@@ -1031,6 +1071,7 @@ int sgpe_graph_steps(sgpe_plan* p, int k, int n_graphs) {
     }
     for (int i = 0; i < n_graphs; ++i) HIP_TRY(hipGraphLaunch(it->second, p->ctx->stream));
     p->last_dur = DUR_OUT;
+    p->pending = true;
     return SGPE_OK;
   });
 }
@@ -1061,6 +1102,7 @@ int sgpe_profile_full_steps(sgpe_plan* p, int n_steps, double* out4) {
         launch_row(p, s, durs[j], j == 0, ev[2 * k], ev[2 * k + 1]);
         ++k;
         p->last_dur = durs[j];
+        p->pending = true;
       }
     }
     hipError_t le = hipGetLastError();
@@ -1207,7 +1249,11 @@ int sgpe_group_run(sgpe_group* g, int k, int check, int n_graphs) {
       it = g->graphs.emplace(key, exec).first;
     }
     for (int i = 0; i < n_graphs; ++i) HIP_TRY(hipGraphLaunch(it->second, g->ctx->stream));
-    for (auto* p : g->plans) p->last_dur = DUR_OUT;
+    // With the energy check the block ends on a sync (normalized k-space).
+    for (auto* p : g->plans) {
+      p->last_dur = DUR_OUT;
+      p->pending = !check;
+    }
     return SGPE_OK;
   });
 }

@@ -125,8 +125,11 @@ void erow_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const InstDev* i, cons
   X(R, PR, 512, 8, 16)                                                                                     \
   X(R, PR, 512, 16, 4) X(R, PR, 512, 16, 8) X(R, PR, 1024, 4, 2) X(R, PR, 1024, 4, 4) X(R, PR, 1024, 8, 2)  \
   X(R, PR, 1024, 8, 4) X(R, PR, 1024, 16, 4) X(R, PR, 1024, 16, 8) X(R, PR, 2048, 4, 2) X(R, PR, 2048, 8, 2) \
-  X(R, PR, 2048, 8, 4) X(R, PR, 2048, 16, 2) X(R, PR, 2048, 16, 4) X(R, PR, 4096, 8, 2) X(R, PR, 4096, 16, 2) \
-  X(R, PR, 4096, 16, 4)
+  X(R, PR, 2048, 8, 4) X(R, PR, 2048, 16, 2) X(R, PR, 2048, 16, 4) X(R, PR, 4096, 8, 2) X(R, PR, 4096, 16, 2)
+// Column shapes whose exchange buffer fits the LDS only in complex64
+// (4 columns of 4096 complex128 points would need 256 KiB).
+#define SGPE_COL_SHAPES_F32_ONLY(X, R, PR) X(R, PR, 4096, 16, 4)
+#define SGPE_COL_SHAPES_F64_ONLY(X, R, PR)
 #define SGPE_ROW_SHAPES(X, R, PR)                                                                          \
   X(R, PR, 64, 2, 2) X(R, PR, 64, 4, 4) X(R, PR, 64, 8, 8) X(R, PR, 64, 8, 16) X(R, PR, 128, 2, 1)          \
   X(R, PR, 128, 2, 2) X(R, PR, 128, 4, 2) X(R, PR, 128, 4, 4) X(R, PR, 128, 8, 4) X(R, PR, 128, 8, 8)       \
@@ -148,9 +151,9 @@ void erow_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const InstDev* i, cons
   X(R, PR, 64, 8, 8) X(R, PR, 128, 8, 8) X(R, PR, 256, 8, 4) X(R, PR, 512, 8, 4) X(R, PR, 1024, 8, 4) \
   X(R, PR, 2048, 8, 4) X(R, PR, 4096, 16, 4)
 
-#define SGPE_DEFINE_REGISTRY(FN, R, PR, DEFCOLS)                                                     \
+#define SGPE_DEFINE_REGISTRY(FN, R, PR, DEFCOLS, COLONLY)                                            \
   namespace sgpe {                                                                                   \
-  static const KEntry k_col[] = {SGPE_COL_SHAPES(SGPE_COL_E, R, PR)};                                \
+  static const KEntry k_col[] = {SGPE_COL_SHAPES(SGPE_COL_E, R, PR) COLONLY(SGPE_COL_E, R, PR)};     \
   static const KEntry k_row[] = {SGPE_ROW_SHAPES(SGPE_ROW_E, R, PR)};                                \
   static const KEntry k_frow[] = {SGPE_DEF_ROWS(SGPE_FROW_E, R, PR)};                                \
   static const KEntry k_fcol[] = {DEFCOLS(SGPE_FCOL_E, R, PR)};                                      \

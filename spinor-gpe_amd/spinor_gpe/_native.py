@@ -61,6 +61,7 @@ def _declare(lib):
     sig = {
         'sgpe_abi_version': (i, []),
         'sgpe_last_error': (ctypes.c_char_p, []),
+        'sgpe_debug_fail_allocs': (i, [i]),
         'sgpe_ctx_create': (i, [i, vp, ctypes.POINTER(vp)]),
         'sgpe_ctx_set_stream': (i, [vp, vp]),
         'sgpe_ctx_synchronize': (i, [vp]),

@@ -132,6 +132,7 @@ class Plan:
                                              self.n_slots, ctypes.byref(handle)))
         self.handle = handle
         self.states = [None] * self.n_slots
+        self.run_pops = [None] * self.n_slots
         if not os.environ.get('SGPE_NO_TUNE_CACHE'):
             from spinor_gpe.engine import autotune  # pylint: disable=import-outside-toplevel
             autotune.apply(self)
@@ -174,7 +175,10 @@ class Plan:
         self.states[slot] = state
 
     def set_run(self, slot, pops=None, conv_tol=0.0):
-        """Start a run: ``pops`` is a float64 (n, 2) device tensor or None."""
+        """Start a run: ``pops`` is a float64 (n, 2) device tensor or None.
+
+        The plan keeps a reference to ``pops`` until the slot's next run, so
+        the device never writes populations into a freed buffer."""
         if pops is None:
             self._call('sgpe_plan_set_run', slot, None, 0, float(conv_tol))
         else:
@@ -182,6 +186,7 @@ class Plan:
                 raise ValueError("pops must be a contiguous float64 tensor on the plan device")
             self._call('sgpe_plan_set_run', slot, ctypes.c_void_p(pops.data_ptr()),
                        int(pops.shape[0]), float(conv_tol))
+        self.run_pops[slot] = pops
 
     def set_active(self, slot, active):
         self._call('sgpe_plan_set_active', slot, int(bool(active)))

@@ -32,6 +32,34 @@ except ImportError:  # pragma: no cover
     _tqdm = None
 
 
+class _DeviceDict(dict):
+    """The reference's ``space`` dict of device tensors (tensor_propagator.py:
+    119-120), holding host arrays and making each device tensor on first
+    access."""
+
+    def __init__(self, host, dev):
+        super().__init__(host)
+        self._dev = dev
+        self._made = set()
+
+    def __getitem__(self, key):
+        val = super().__getitem__(key)
+        if key not in self._made:
+            val = torch.tensor(val, device=self._dev)
+            super().__setitem__(key, val)
+            self._made.add(key)
+        return val
+
+    def get(self, key, default=None):
+        return self[key] if key in self else default
+
+    def values(self):
+        return [self[k] for k in self]
+
+    def items(self):
+        return [(k, self[k]) for k in self]
+
+
 class TensorPropagator:
     """GPU propagator of the pseudospin-1/2 GPE (tensor_propagator.py:11-61)."""
 
@@ -62,17 +90,20 @@ class TensorPropagator:
         self.atom_num = spin.atom_num
         self.is_coupling = spin.is_coupling
         self.g_sc = spin.g_sc
-        self.kin_eng_spin = ttools.to_tensor(spin.kin_eng_spin, dev=self._dev)
-        self.pot_eng_spin = ttools.to_tensor(spin.pot_eng_spin, dev=self._dev)
+        # The operator grids stay on the host: the engine builds its own
+        # phase tables from them, and only the end-of-run energy reads them.
+        # The reference's device copies (tensor_propagator.py:114-129) are
+        # made on first access of the attributes below.
+        self._kin_host = [np.asarray(k, dtype=np.float64) for k in spin.kin_eng_spin]
+        self._pot_host = [np.asarray(v, dtype=np.float64) for v in spin.pot_eng_spin]
         keys_space = ['dr', 'dk', 'x_mesh', 'y_mesh', 'dv_r', 'dv_k']
-        self.space = {k: torch.tensor(spin.space[k], device=self._dev) for k in keys_space}
-        self.coupling = ttools.to_tensor(spin.coupling, dev=self._dev)
+        self._space_host = {k: np.asarray(spin.space[k]) for k in keys_space}
+        self._coupling_host = np.asarray(spin.coupling, dtype=np.float64)
+        self.space = _DeviceDict(self._space_host, self._dev)
+        self._device_copies = {}
         # pylint: disable=invalid-name
         self.kL_recoil = spin.kL_recoil
-        if spin.rot_coupling:
-            self.expon = torch.tensor(0.0)
-        else:
-            self.expon = 2 * self.kL_recoil * self.space['x_mesh']
+        self._rot_coupling = spin.rot_coupling
         if self.is_sampling:
             assert self.n_steps % n_samples == 0, (
                 f"The number of samples requested {n_samples} does not evenly "
@@ -92,6 +123,35 @@ class TensorPropagator:
         self._state = self._plan.new_state(spin.psik)
         self._plan.set_slot(0, spec, self._state)
         self._pending = False  # True while the state awaits its closing half-step
+
+    # ----------------------------------------- operator grids (on demand)
+    def _device_copy(self, key, make):
+        if key not in self._device_copies:
+            self._device_copies[key] = make()
+        return self._device_copies[key]
+
+    @property
+    def kin_eng_spin(self):
+        """Kinetic energy grids as device tensors (tensor_propagator.py:114)."""
+        return self._device_copy('kin', lambda: ttools.to_tensor(self._kin_host, dev=self._dev))
+
+    @property
+    def pot_eng_spin(self):
+        """Potential energy grids as device tensors (tensor_propagator.py:115)."""
+        return self._device_copy('pot', lambda: ttools.to_tensor(self._pot_host, dev=self._dev))
+
+    @property
+    def coupling(self):
+        """Coupling grid as a device tensor (tensor_propagator.py:121)."""
+        return self._device_copy('coupling', lambda: ttools.to_tensor(self._coupling_host, dev=self._dev))
+
+    @property
+    def expon(self):
+        """Coupling phase: 0 in the rotated frame, else 2 kL x
+        (tensor_propagator.py:126-129)."""
+        if self._rot_coupling:
+            return torch.tensor(0.0)
+        return self._device_copy('expon', lambda: 2 * self.kL_recoil * self.space['x_mesh'])
 
     # ------------------------------------------------------------- state
     @property
@@ -199,6 +259,11 @@ class TensorPropagator:
         pops['vals'][:] = pops_dev[:done].cpu().numpy()
         self._plan.set_active(0, True)
         egp = self._plan.energy()[0].cpu().numpy()
+        # Detach the population buffer before it goes out of scope: the run
+        # state would otherwise keep its address, and later full_step() /
+        # single_step() calls would write populations into freed memory.
+        self._plan.set_run(0, None, conv_tol=0.0)
+        del pops_dev
 
         psik = ttools.to_numpy(self.psik)
         psi = ttools.to_numpy(ttools.ifft_2d(self.psik, self.space['dr']))
@@ -220,33 +285,42 @@ class TensorPropagator:
 
     # ------------------------------------------------------------ energy
     def eng_expect(self, psik, psi=None):
-        """End-of-run energy list ``[E_tot, E_kin, E_pot, E_int]`` on the host.
-
-        Host NumPy evaluation with the reference's functional and conventions
-        (tensor_propagator.py:273-324): finite-difference gradients of the
-        square-root density and of the unwrapped, masked phase (axis 0 as the
-        'x' derivative in the recoil term), no 1/2 on the interaction terms,
-        the coupling term without its phase, and no area element. ``psi``
-        may pass the already transformed real-space state.
-        """
+        """End-of-run energy list ``[E_tot, E_kin, E_pot, E_int]`` on the host
+        (tensor_propagator.py:273-324); see :func:`eng_expect_arrays`.
+        ``psi`` may pass the already transformed real-space state."""
         assert len(psik) == 2, "Requires two spinor components to calculate the energy."
-        if not isinstance(psik[0], np.ndarray):
-            psik = ttools.to_numpy(psik)
-        delta_r = ttools.to_numpy(self.space['dr'])
-        if psi is None:
-            psi = ttools.ifft_2d(psik, delta_r)
-        dens = ttools.density(psi)
-        root = [np.sqrt(d) for d in dens]
-        ph = ttools.phase(psi, uwrap=True, dens=dens)
-        ph_ax0 = [g[0] for g in ttools.grad(ph, delta_r)]
-        ph_gsq = ttools.grad_sq(ph, delta_r)
-        kin = (sum(ttools.grad_sq(root, delta_r))
-               + sum([d * p for d, p in zip(dens, ph_gsq)])
-               + sum([d * p for d, p in zip(dens, ph_ax0)]) * 2 * self.kL_recoil * self.is_coupling) / 2
-        pot = sum([d * v for d, v in zip(dens, ttools.to_numpy(self.pot_eng_spin))])
-        int_e = (self.g_sc['uu'] * dens[0] ** 2 + self.g_sc['dd'] * dens[1] ** 2
-                 + self.g_sc['ud'] * dens[0] * dens[1])
-        cpl = ((np.conj(psi[0]) * psi[1] + np.conj(psi[1]) * psi[0])
-               * ttools.to_numpy(self.coupling) / 2)
-        total = np.real((kin + pot + int_e + cpl).sum())
-        return [total, np.real(kin).sum(), np.real(pot).sum(), np.real(int_e).sum()]
+        return eng_expect_arrays(psik, self._space_host['dr'], self._pot_host, self._coupling_host,
+                                 self.g_sc, self.kL_recoil, self.is_coupling, psi=psi)
+
+
+def eng_expect_arrays(psik, delta_r, pot_eng_spin, coupling, g_sc, kl_recoil, is_coupling, psi=None):
+    """The reference's end-of-run energy of a k-space spinor, host NumPy.
+
+    Same functional and conventions as tensor_propagator.py:293-324: the
+    real-space state by the NumPy ``ifft_2d``; finite-difference gradients
+    (``np.gradient(f, dr[0], dr[1])``: dr[0] is the spacing of axis 0) of
+    the square-root densities and of the unwrapped phases, masked to 0 where
+    the density is below 1e-6 of its maximum (tensor_tools.py:528-539); the
+    recoil term uses the axis-0 phase derivative; interaction terms have no
+    1/2; the coupling term omits the coupling phase; no area element.
+    Returns ``[E_tot, sum kin, sum pot, sum int]``.
+    """
+    if not isinstance(psik[0], np.ndarray):
+        psik = ttools.to_numpy(psik)
+    delta_r = np.asarray(ttools.to_numpy(delta_r) if not isinstance(delta_r, np.ndarray) else delta_r)
+    if psi is None:
+        psi = ttools.ifft_2d(psik, delta_r)
+    dens = ttools.density(psi)
+    root = [np.sqrt(d) for d in dens]
+    ph = ttools.phase(psi, uwrap=True, dens=dens)
+    ph_ax0 = [g[0] for g in ttools.grad(ph, delta_r)]
+    ph_gsq = ttools.grad_sq(ph, delta_r)
+    kin = (sum(ttools.grad_sq(root, delta_r))
+           + sum([d * p for d, p in zip(dens, ph_gsq)])
+           + sum([d * p for d, p in zip(dens, ph_ax0)]) * 2 * kl_recoil * is_coupling) / 2
+    pot = sum([d * np.asarray(v) for d, v in zip(dens, pot_eng_spin)])
+    int_e = (g_sc['uu'] * dens[0] ** 2 + g_sc['dd'] * dens[1] ** 2
+             + g_sc['ud'] * dens[0] * dens[1])
+    cpl = (np.conj(psi[0]) * psi[1] + np.conj(psi[1]) * psi[0]) * np.asarray(coupling) / 2
+    total = np.real((kin + pot + int_e + cpl).sum())
+    return [total, np.real(kin).sum(), np.real(pot).sum(), np.real(int_e).sum()]

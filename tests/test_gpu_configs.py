@@ -4,14 +4,18 @@
   against the CPU oracle; state to 1e-10 (relative L2), ground-state energy
   and chemical potential to 1e-12.
 * 512^2 real-time Raman run (lab frame): the first 100 steps against the CPU
-  oracle (1e-11), then the full 20000 steps through size-independent
-  properties: atom number conserved exactly, GP energy conserved to the
-  splitting error, complex64 (FP64 reductions) within stated fidelity and
-  energy bounds of complex128.
+  oracle (1e-11); the whole 20000-step complex128 run against the committed
+  oracle fixture of the same run (state to 1e-10); and the full run through
+  size-independent properties: atom number conserved exactly, GP energy
+  conserved to the splitting error, complex64 (FP64 reductions) within stated
+  fidelity and energy bounds of complex128.
 * 2048^2 real-time run with 64 vortices: 3 steps against the CPU oracle
   (1e-11), then 10000 steps in complex64 and complex128: fidelity and energy
   drift of each reported and bounded.
 """
+
+import importlib.util
+import os
 
 import numpy as np
 import pytest
@@ -74,6 +78,53 @@ def test_c1_prefix_matches_oracle(gpu, c1_initial):
     oracle.run(100)
     res, _ = ps.real(1e-3, 100, 'cuda', progress=False)
     assert rel(res.psik, oracle.psik_numpy()) <= 1e-11
+
+
+GOLDEN_C1 = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden', 'c1_oracle_20000.npz')
+
+
+def _fixture_module():
+    spec = importlib.util.spec_from_file_location(
+        'make_c1_fixture', os.path.join(os.path.dirname(GOLDEN_C1), 'make_c1_fixture.py'))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_c1_full_run_matches_oracle_fixture(gpu, tmp_path, monkeypatch):
+    """The whole 20000-step 512^2 Raman run (complex128) against the oracle's
+    run of the same inputs (tests/golden/make_c1_fixture.py): at steps 100,
+    1000, 5000, 10000 and 20000 the state on a stride-16 sublattice to 1e-10
+    (relative L2) and its projections on 8 seeded +-1 vectors to 4e-10 of
+    ||psik|| (each projection of an error e has mean square ||e||^2, so this
+    bounds the full state's error), the populations every 10th step to 1e-11
+    of N_at, and the final GP energy and chemical potential to 1e-10."""
+    monkeypatch.setenv('SPINOR_GPE_DATA', str(tmp_path))
+    print("[configs] 512^2 Raman real time, 20000 steps vs the oracle fixture, c128", flush=True)
+    fx = np.load(GOLDEN_C1)
+    mk = _fixture_module()
+    ps0, o0 = mk.c0_ground_oracle()  # the oracle's ground state, as the fixture's
+    assert rel(np.stack(o0.psik_numpy())[:, ::8, ::8], fx['c0_sub']) <= 1e-12
+    ps = wl.c1_spinor(o0.psi_numpy(), ps0.space, name='c1-golden-gpu')
+    init = np.stack(ps.psik)
+    assert rel(init[:, ::16, ::16], fx['init_sub']) <= 1e-12
+    r = mk.rademacher(init.size, int(fx['proj_k']), int(fx['proj_seed']))
+    done, pops = 0, []
+    for k, ck in enumerate(fx['ck_steps']):
+        res, _ = ps.real(1e-3, int(ck) - done, 'cuda', progress=False, block_steps=100)
+        pops.append(res.pops['vals'])
+        done = int(ck)
+        st = np.stack(res.psik)
+        sub = rel(st[:, ::16, ::16], fx['ck_sub'][k])
+        proj = np.max(np.abs(mk.project(st, r) - fx['ck_proj'][k])) / fx['ck_norm'][k]
+        print(f"[configs] c1 step {ck}: sublattice rel-L2 {sub:.3e}, projections {proj:.3e}", flush=True)
+        assert sub <= 1e-10, (ck, sub)
+        assert proj <= 4e-10, (ck, proj)
+    pops = np.concatenate(pops)[int(fx['pops_every']) - 1::int(fx['pops_every'])]
+    assert np.max(np.abs(pops - fx['pops'])) <= 1e-11 * ps.atom_num
+    e_tot, mu = fx['gp'][0], fx['gp'][6]
+    assert res.energy_gp['E_tot'] == pytest.approx(e_tot, rel=1e-10)
+    assert res.energy_gp['mu'] == pytest.approx(mu, rel=1e-10)
 
 
 def test_c1_full_run_invariants(gpu, c1_initial):
