@@ -150,7 +150,8 @@ int sgpe_plan_set_run(sgpe_plan* plan, int slot, double* pops, int cap, double c
 
 /* Per-slot run state, read back to HOST memory (synchronizes the stream):
  * out3[3*slot + {0,1,2}] = {active, converged, full steps in this run};
- * energy8 (optional) = the last sgpe_energy results per slot. */
+ * energy8 (optional) = the last sgpe_energy results per slot. Returns an
+ * error if a persistent launch stopped (see sgpe_plan_set_persistent). */
 int sgpe_plan_status(sgpe_plan* plan, int* out3, double* energy8);
 
 /* Enable (1) or disable (0) a slot; disabled slots are skipped by every
@@ -169,6 +170,36 @@ int sgpe_list_shapes(int pass, int precision, int n, int* out_pairs, int cap);
 
 /* Current shapes: out4 = {row_e, row_rows, col_e, col_w}. */
 int sgpe_plan_get_tuning(const sgpe_plan* plan, int* out4);
+
+/* Run blocks of full steps (sgpe_full_steps, sgpe_graph_steps, sgpe_group_run)
+ * as ONE launch of a persistent kernel per block instead of six pass
+ * launches per step (on = 1), for the loop of prop_loop (tensor_propagator.py:
+ * 185-194). Used where a persistent kernel is compiled for the plan's
+ * shapes (square grids), all its workgroups can be resident at once, and
+ * every loaded slot has separable operators with no or uniform coupling;
+ * elsewhere the pass launches are used. Results are those of the pass
+ * launches. wait_ticks (<= 0: default, 2 s) bounds each wait of one
+ * workgroup for another, in ticks of the 100 MHz device wall clock: past it
+ * the launch stops (every workgroup returns) and sgpe_plan_status /
+ * sgpe_plan_check report the error. */
+int sgpe_plan_set_persistent(sgpe_plan* plan, int on, long long wait_ticks);
+
+/* Shapes with a compiled persistent kernel for n x n grids: writes up to cap
+ * quadruples {row_e, row_rows, col_e, col_w} and returns the number
+ * available (>= 0) or a negative error. */
+int sgpe_list_persistent(int precision, int n, int* out4, int cap);
+
+/* out3 = {requested, used for the next block, grid size (0: no persistent
+ * kernel for the current shapes)}. */
+int sgpe_plan_get_persistent(const sgpe_plan* plan, int* out3);
+
+/* Synchronizes the stream; returns an error if a persistent launch of the
+ * plan stopped on its wait bound since the states were last set. */
+int sgpe_plan_check(sgpe_plan* plan);
+
+/* Measurement: n_steps full steps as one persistent launch whose execution
+ * is timestamped by a hipEvent pair; *ms = its duration. Synchronizes. */
+int sgpe_profile_persistent(sgpe_plan* plan, int n_steps, double* ms);
 
 /* Cache policy of the two passes' state reads and writes: 1 = non-temporal
  * (streaming) accesses, 0 = normal. Default: streaming for single-slot plans,

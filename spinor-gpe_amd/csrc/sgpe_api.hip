@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/sgpe.h"
+#include "sgpe_persist.h"
 #include "sgpe_registry.h"
 
 using namespace sgpe;
@@ -144,10 +145,24 @@ static int set_lds_attr(const void* kernel, size_t bytes) {
 
 static bool pow2_in_range(int n) { return n >= 64 && n <= 4096 && (n & (n - 1)) == 0; }
 
+// Bound of one wait of the persistent kernel: 2 s of the 100 MHz wall clock
+// (a pass takes microseconds to a few milliseconds).
+static const long long kDefaultWaitTicks = 200000000LL;
+
+static const PEntry* find_persist(int prec, int n, int re, int rr, int ce, int cw) {
+  const PTable t = prec == SGPE_C128 ? persist_registry_f64() : persist_registry_f32();
+  for (int i = 0; i < t.count; ++i) {
+    const PEntry& e = t.entries[i];
+    if (e.n == n && e.row_e == re && e.row_rows == rr && e.col_e == ce && e.col_w == cw) return &e;
+  }
+  return nullptr;
+}
+
 // ----------------------------------------------------------------- context
 
 struct sgpe_ctx {
   int device = 0;
+  int n_cu = 0;                      // compute units (residency check of persistent launches)
   hipStream_t stream = nullptr;
   hipStream_t cap_stream = nullptr;  // private stream used only for graph capture
   std::map<int, void*> tw;           // stage-major twiddles, key (prec*8192 + n)*32 + e
@@ -273,6 +288,15 @@ struct sgpe_plan {
   int e_col_e = 8, e_col_w = 4, e_row_e = 8, e_row_rows = 1;
   const void* etw_x = nullptr;  // twiddles of the energy kernels' shapes
   const void* etw_y = nullptr;
+  // Persistent step kernel (sgpe_persist.h): preference, the entry for the
+  // current shapes (null: none compiled or not resident), its grid and LDS.
+  bool persist = false;
+  const PEntry* pk = nullptr;
+  int pgrid = 0;
+  size_t plds = 0;
+  long long wait_ticks = kDefaultWaitTicks;
+  PersistSync* d_sync = nullptr;  // [nslots]
+  unsigned* d_pstatus = nullptr;  // [PST_WORDS]
 };
 
 static void drop_graphs(sgpe_plan* p) {
@@ -309,6 +333,26 @@ static int plan_select(sgpe_plan* p, int row_e, int row_rows, int col_e, int col
   p->row_rows = row_rows;
   p->col_e = col_e;
   p->col_w = col_w;
+  // Persistent kernel of these shapes, if compiled and if every workgroup of
+  // its grid can be resident at once (occupancy of the kernel x CUs).
+  p->pk = nullptr;
+  p->pgrid = 0;
+  const PEntry* pe = p->nx == p->ny && p->nslots <= kPersistMaxSlots
+                         ? find_persist(p->prec, p->nx, row_e, row_rows, col_e, col_w)
+                         : nullptr;
+  if (pe) {
+    const size_t lds = std::max(rl, cl);
+    const int block = row_rows * (p->nx / row_e);
+    int per_cu = 0;
+    if ((rc = set_lds_attr(pe->kernel, lds))) return rc;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pe->kernel, block, lds));
+    const int items = std::max(2 * (p->nx / col_w), p->ny / row_rows) * p->nslots;
+    if (per_cu > 0 && p->ctx->n_cu > 0) {
+      p->pk = pe;
+      p->plds = lds;
+      p->pgrid = std::min(per_cu * p->ctx->n_cu, items);
+    }
+  }
   drop_graphs(p);
   return SGPE_OK;
 }
@@ -350,6 +394,53 @@ static void issue_full_step(sgpe_plan* p, hipStream_t s) {
   issue_sub_step(p, s, DUR_OUT, 1);
   issue_sub_step(p, s, DUR_IN, 0);
   issue_sub_step(p, s, DUR_OUT, 0);
+}
+
+// Whether n full steps run as one launch of the persistent kernel: asked
+// for, compiled and resident for the current shapes, and every loaded slot
+// on the fast operator path.
+static bool use_persist(const sgpe_plan* p) { return p->persist && p->pk && !p->dense_kin && !p->general_row; }
+
+// n full steps as one persistent launch, after a reset of its counters.
+static void issue_persist(sgpe_plan* p, hipStream_t s, int n, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+  hipLaunchKernelGGL(persist_reset_kernel, dim3(1), dim3(256), 0, s, p->d_sync, p->nslots, p->d_pstatus, 0);
+  PersistArgs a;
+  a.insts = p->d_inst;
+  a.runs = p->d_run;
+  a.direct_psi = direct_state(p);
+  a.nslots = p->nslots;
+  a.n_steps = n;
+  a.d_first_prev = p->last_dur;
+  a.col_stream = p->col_stream;
+  a.row_stream = p->row_stream;
+  a.tw_x = p->tw_x;
+  a.tw_y = p->tw_y;
+  a.sync = p->d_sync;
+  a.status = p->d_pstatus;
+  a.wait_ticks = p->wait_ticks;
+  const int block = p->row_rows * (p->nx / p->row_e);
+  ((PFn)p->pk->launch)(dim3(p->pgrid), dim3(block), p->plds, s, e0, e1, a);
+  p->last_dur = DUR_OUT;
+  p->pending = true;
+}
+
+static const char* const kAbortMsg =
+    "a persistent step launch timed out waiting for another workgroup and stopped; the plan's states are "
+    "invalid until they are set again (sgpe_plan_set_slot / sgpe_plan_set_state)";
+
+// Clears the abort flag of the persistent kernel (new states bound).
+static void clear_persist_abort(sgpe_plan* p, hipStream_t s) {
+  hipLaunchKernelGGL(persist_reset_kernel, dim3(1), dim3(256), 0, s, p->d_sync, p->nslots, p->d_pstatus, 1);
+}
+
+// n full steps: one persistent launch, or six pass launches per step.
+static void issue_steps(sgpe_plan* p, hipStream_t s, int n) {
+  if (n <= 0) return;
+  if (use_persist(p)) {
+    issue_persist(p, s, n);
+    return;
+  }
+  for (int i = 0; i < n; ++i) issue_full_step(p, s);
 }
 
 static void issue_sync(sgpe_plan* p, hipStream_t s) {
@@ -630,6 +721,7 @@ static int plan_load_slot(sgpe_plan* p, int slot, const sgpe_inst_desc* d, void*
   HIP_TRY(hipMemcpyAsync(p->d_inst + slot, &h, sizeof(h), hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(config_run_kernel, dim3(1), dim3(64), 0, s, p->d_run, slot, (double*)nullptr, 0, 1, 1,
                      (int)FORM_KSPACE, 0.0);
+  clear_persist_abort(p, s);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(s));  // staging buffer goes out of scope; old tables unused from here on
   if (fresh) {
@@ -690,6 +782,7 @@ int sgpe_ctx_create(int device, void* stream, sgpe_ctx** out) {
     HIP_TRY(hipSetDevice(device));
     sgpe_ctx* c = new sgpe_ctx();
     c->device = device;
+    if (hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) c->n_cu = 0;
     c->stream = (hipStream_t)stream;
     if (hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking) != hipSuccess) {
       delete c;
@@ -865,6 +958,12 @@ int sgpe_plan_create(sgpe_ctx* ctx, int precision, int ny, int nx, int n_slots, 
       rc = fail(SGPE_E_HIP, "run state initialization failed");
     if (!rc && hipMemset(p->d_inst, 0, n_slots * sizeof(InstDev)) != hipSuccess)
       rc = fail(SGPE_E_HIP, "slot table initialization failed");
+    if (!rc && dev_alloc((void**)&p->d_sync, n_slots * sizeof(PersistSync)) != hipSuccess)
+      rc = fail(SGPE_E_NOMEM, "persistent counters allocation failed");
+    if (!rc && dev_alloc((void**)&p->d_pstatus, PST_WORDS * sizeof(unsigned)) != hipSuccess)
+      rc = fail(SGPE_E_NOMEM, "persistent status allocation failed");
+    if (!rc && hipMemset(p->d_pstatus, 0, PST_WORDS * sizeof(unsigned)) != hipSuccess)
+      rc = fail(SGPE_E_HIP, "persistent status initialization failed");
     if (rc) {
       sgpe_plan_destroy(p);
       return rc;
@@ -884,6 +983,8 @@ void sgpe_plan_destroy(sgpe_plan* p) {
   if (p->scratch) (void)hipFree(p->scratch);
   if (p->d_inst) (void)hipFree(p->d_inst);
   if (p->d_run) (void)hipFree(p->d_run);
+  if (p->d_sync) (void)hipFree(p->d_sync);
+  if (p->d_pstatus) (void)hipFree(p->d_pstatus);
   delete p;
 }
 
@@ -907,6 +1008,7 @@ int sgpe_plan_set_state(sgpe_plan* p, int slot, void* psik) {
     HIP_TRY(hipMemcpyAsync(&p->d_inst[slot].psi, &h.psi, sizeof(void*), hipMemcpyHostToDevice, p->ctx->stream));
     hipLaunchKernelGGL(config_run_kernel, dim3(1), dim3(64), 0, p->ctx->stream, p->d_run, slot, (double*)nullptr, 0, 1, 1,
                        (int)FORM_KSPACE, 0.0);
+    clear_persist_abort(p, p->ctx->stream);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(p->ctx->stream));
     return SGPE_OK;
@@ -930,8 +1032,11 @@ int sgpe_plan_status(sgpe_plan* p, int* out3, double* energy8) {
     if (!p || !out3) return fail(SGPE_E_ARG, "bad arguments");
     HIP_TRY(hipSetDevice(p->ctx->device));
     std::vector<RunDev> h(p->nslots);
+    unsigned pst[PST_WORDS] = {0, 0};
     HIP_TRY(hipMemcpyAsync(h.data(), p->d_run, p->nslots * sizeof(RunDev), hipMemcpyDeviceToHost, p->ctx->stream));
+    HIP_TRY(hipMemcpyAsync(pst, p->d_pstatus, sizeof(pst), hipMemcpyDeviceToHost, p->ctx->stream));
     HIP_TRY(hipStreamSynchronize(p->ctx->stream));
+    if (pst[PST_ABORT]) return fail(SGPE_E_HIP, kAbortMsg);
     for (int i = 0; i < p->nslots; ++i) {
       out3[3 * i + 0] = h[i].active;
       out3[3 * i + 1] = h[i].converged;
@@ -1005,6 +1110,86 @@ int sgpe_plan_get_streaming(const sgpe_plan* p, int* out2) {
   });
 }
 
+int sgpe_list_persistent(int precision, int n, int* out4, int cap) {
+  return guarded([&]() -> int {
+    if ((precision != SGPE_C128 && precision != SGPE_C64) || cap < 0) return fail(SGPE_E_ARG, "bad arguments");
+    const PTable t = precision == SGPE_C128 ? persist_registry_f64() : persist_registry_f32();
+    int count = 0;
+    for (int i = 0; i < t.count; ++i) {
+      const PEntry& e = t.entries[i];
+      if (e.n != n) continue;
+      if (out4 && count < cap) {
+        out4[4 * count + 0] = e.row_e;
+        out4[4 * count + 1] = e.row_rows;
+        out4[4 * count + 2] = e.col_e;
+        out4[4 * count + 3] = e.col_w;
+      }
+      ++count;
+    }
+    return count;
+  });
+}
+
+int sgpe_plan_set_persistent(sgpe_plan* p, int on, long long wait_ticks) {
+  return guarded([&]() -> int {
+    if (!p) return fail(SGPE_E_ARG, "plan is NULL");
+    p->persist = on != 0;
+    p->wait_ticks = wait_ticks > 0 ? wait_ticks : kDefaultWaitTicks;
+    drop_graphs(p);
+    return SGPE_OK;
+  });
+}
+
+int sgpe_plan_get_persistent(const sgpe_plan* p, int* out3) {
+  return guarded([&]() -> int {
+    if (!p || !out3) return fail(SGPE_E_ARG, "bad arguments");
+    out3[0] = p->persist ? 1 : 0;
+    out3[1] = use_persist(p) ? 1 : 0;
+    out3[2] = p->pk ? p->pgrid : 0;
+    return SGPE_OK;
+  });
+}
+
+int sgpe_plan_check(sgpe_plan* p) {
+  return guarded([&]() -> int {
+    if (!p) return fail(SGPE_E_ARG, "plan is NULL");
+    HIP_TRY(hipSetDevice(p->ctx->device));
+    unsigned pst[PST_WORDS] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(pst, p->d_pstatus, sizeof(pst), hipMemcpyDeviceToHost, p->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(p->ctx->stream));
+    if (pst[PST_ABORT]) return fail(SGPE_E_HIP, kAbortMsg);
+    return SGPE_OK;
+  });
+}
+
+int sgpe_profile_persistent(sgpe_plan* p, int n_steps, double* ms_out) {
+  return guarded([&]() -> int {
+    int rc = plan_ready(p);
+    if (rc) return rc;
+    if (n_steps <= 0 || !ms_out) return fail(SGPE_E_ARG, "bad arguments");
+    if (!use_persist(p)) return fail(SGPE_E_UNSUPPORTED, "the persistent path is off or not available for this plan");
+    HIP_TRY(hipSetDevice(p->ctx->device));
+    hipStream_t s = p->ctx->stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+      if (e0) (void)hipEventDestroy(e0);
+      return fail(SGPE_E_HIP, "event creation failed");
+    }
+    issue_persist(p, s, n_steps, e0, e1);
+    hipError_t le = hipGetLastError();
+    hipError_t se = hipStreamSynchronize(s);
+    float ms = 0.f;
+    hipError_t te = (le == hipSuccess && se == hipSuccess) ? hipEventElapsedTime(&ms, e0, e1) : hipSuccess;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (le != hipSuccess) return fail(SGPE_E_HIP, std::string("launch failed: ") + hipGetErrorString(le));
+    if (se != hipSuccess) return fail(SGPE_E_HIP, std::string("synchronize failed: ") + hipGetErrorString(se));
+    if (te != hipSuccess) return fail(SGPE_E_HIP, "event timing failed");
+    *ms_out = ms;
+    return SGPE_OK;
+  });
+}
+
 int sgpe_plan_get_tuning(const sgpe_plan* p, int* out4) {
   return guarded([&]() -> int {
     if (!p || !out4) return fail(SGPE_E_ARG, "bad arguments");
@@ -1034,7 +1219,7 @@ int sgpe_full_steps(sgpe_plan* p, int n_steps) {
     if (rc) return rc;
     if (n_steps < 0) return fail(SGPE_E_ARG, "n_steps < 0");
     HIP_TRY(hipSetDevice(p->ctx->device));
-    for (int i = 0; i < n_steps; ++i) issue_full_step(p, p->ctx->stream);
+    issue_steps(p, p->ctx->stream, n_steps);
     HIP_TRY(hipGetLastError());
     return SGPE_OK;
   });
@@ -1055,7 +1240,7 @@ int sgpe_graph_steps(sgpe_plan* p, int k, int n_graphs) {
       hipGraphExec_t ge = nullptr;
       hipStream_t cs = p->ctx->cap_stream;
       HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-      for (int i = 0; i < k; ++i) issue_full_step(p, cs);
+      issue_steps(p, cs, k);
       hipError_t le = hipGetLastError();
       hipError_t ce = hipStreamEndCapture(cs, &g);
       if (le != hipSuccess || ce != hipSuccess) {
@@ -1224,14 +1409,20 @@ int sgpe_group_run(sgpe_group* g, int k, int check, int n_graphs) {
       hipStream_t cs = g->ctx->cap_stream;
       hipGraph_t graph = nullptr;
       hipGraphExec_t exec = nullptr;
+      // Persistent launches each take the whole GPU (their workgroups must
+      // all be resident), so the plans that use one run one after another
+      // (a chain of branches); the others run concurrently.
       HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
       hipError_t e = hipEventRecord(g->fork, cs);
+      hipEvent_t chain = g->fork;
       for (size_t i = 0; i < g->plans.size() && e == hipSuccess; ++i) {
-        e = hipStreamWaitEvent(g->side[i], g->fork, 0);
+        const bool persistent = use_persist(g->plans[i]);
+        e = hipStreamWaitEvent(g->side[i], persistent ? chain : g->fork, 0);
         if (e != hipSuccess) break;
-        for (int j = 0; j < k; ++j) issue_full_step(g->plans[i], g->side[i]);
+        issue_steps(g->plans[i], g->side[i], k);
         if (check) issue_energy(g->plans[i], g->side[i], nullptr);
         e = hipEventRecord(g->join[i], g->side[i]);
+        if (persistent) chain = g->join[i];
         if (e == hipSuccess) e = hipStreamWaitEvent(cs, g->join[i], 0);
       }
       hipError_t le = hipGetLastError();

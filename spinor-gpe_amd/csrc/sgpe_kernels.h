@@ -158,13 +158,58 @@ template <typename C> __device__ __forceinline__ void st_stream(C* p, C v) {
   __builtin_nontemporal_store(w, reinterpret_cast<V2*>(p));
 }
 
+// Device-coherent accesses (cache policy SC1: device scope) to data that
+// other workgroups of the same launch write or read, through a buffer
+// resource over [base, base + bytes): writes reach the device's coherence
+// point before the wave's vmcnt drains, and reads do not return lines an
+// L2 of another XCD may hold stale, so workgroups of one launch exchange
+// data without cache-wide writebacks or invalidations. Out-of-range offsets
+// read 0 and drop writes (the resource's bound). 0x00020000 is the
+// resource's third word for gfx9 targets (raw buffer, 32-bit data format).
+constexpr int kCpolSc1 = 16;
+struct CoherentBuf {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ CoherentBuf(void* base, uint32_t bytes)
+      : r(__builtin_amdgcn_make_buffer_rsrc(base, 0, (int)bytes, 0x00020000)) {}
+  template <typename C> __device__ __forceinline__ C load(uint32_t off) const {
+    if constexpr (sizeof(C) == 16) {
+      typedef unsigned int V4 __attribute__((ext_vector_type(4)));
+      const V4 w = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, kCpolSc1);
+      return __builtin_bit_cast(C, w);
+    } else {
+      typedef unsigned int V2 __attribute__((ext_vector_type(2)));
+      const V2 w = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, kCpolSc1);
+      return __builtin_bit_cast(C, w);
+    }
+  }
+  template <typename C> __device__ __forceinline__ void store(uint32_t off, C v) const {
+    if constexpr (sizeof(C) == 16) {
+      typedef unsigned int V4 __attribute__((ext_vector_type(4)));
+      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V4, v), r, (int)off, 0, kCpolSc1);
+    } else {
+      typedef unsigned int V2 __attribute__((ext_vector_type(2)));
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, (int)off, 0, kCpolSc1);
+    }
+  }
+};
+// Access modes of the pass bodies (CM): device-coherent state and partial
+// sum reads (CM_LOAD) and writes (CM_STORE); 0 = ordinary accesses.
+enum { CM_LOAD = 1, CM_STORE = 2 };
+__device__ __forceinline__ double ld_coherent(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_coherent(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Sum of n doubles by one wave: lane l adds entries l, l + 64, ..., then a
 // butterfly across the lanes; every lane gets the same total, and every wave
-// that calls it sums in the same order.
+// that calls it sums in the same order. COH: device-coherent reads.
+template <bool COH = false>
 __device__ __forceinline__ double wave_sum_strided(const double* __restrict__ p, int n) {
   const int lane = threadIdx.x & 63;
   double v = 0.0;
-  for (int i = lane; i < n; i += 64) v += p[i];
+  for (int i = lane; i < n; i += 64) v += COH ? ld_coherent(p + i) : p[i];
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
@@ -222,13 +267,16 @@ __device__ __forceinline__ void col_kinetic(C (&a)[E], int t, int nx, int jx, co
   }
 }
 
-// DENSE: some slot has a dense kinetic array (the kernel then also handles
-// table slots); otherwise every slot uses the separable tables.
-template <typename R, int N, int E, int W, bool DENSE>
-__global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __restrict__ insts,
-                                                             RunDev* __restrict__ runs, void* direct_psi,
-                                                             int direct_n, const typename Cx<R>::T* __restrict__ tws,
-                                                             int mode, int d_prev, int d_cur, int stream) {
+// One column tile of one component (the body of col_pass_kernel and of the
+// persistent step kernel). `gate` is called once every global load of the
+// prologue has been issued: it returns -1 to drop the tile (inactive slot),
+// else whether the stored state is already in k-space (1) or hybrid (0).
+// DENSE: the plan has some slot with a dense kinetic array (the code then
+// also handles table slots); otherwise every slot uses the separable tables.
+template <typename R, int N, int E, int W, bool DENSE, int CM, class Gate>
+__device__ __forceinline__ void col_item(const InstDev& P, typename Cx<R>::T* __restrict__ psi, int nx,
+                                         const typename Cx<R>::T* __restrict__ tws, int mode, int d_prev,
+                                         int d_cur, int stream, int comp, int tile, double* smem, Gate gate) {
   typedef typename Cx<R>::T C;
   typedef LdsView<W, 1 << 30, 0> V;
   constexpr int T = N / E;
@@ -239,20 +287,6 @@ __global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __re
   constexpr bool kKy = kTw && kData + 3 * (size_t)N * sizeof(C) <= kLdsBytes;
   constexpr size_t kBase = kData + (kTw ? (size_t)N * sizeof(C) : 0) + (kKy ? 2 * (size_t)N * sizeof(C) : 0);
   constexpr bool kSumSep = col_sum_separate(kBase);
-  extern __shared__ double smem[];
-  const int inst = blockIdx.z;
-  const InstDev& P = insts[inst];
-  // A single-slot plan passes its state's address and row length as kernel
-  // arguments (direct_psi): the state loads are issued at once and the
-  // active flag is checked after them. Otherwise the address and length are
-  // read together with the active flag, so the state loads wait for one
-  // round of scalar loads, not two.
-  const bool direct = direct_psi != nullptr;
-  C* const psi = reinterpret_cast<C*>(direct ? direct_psi : P.psi);
-  const int nx = direct ? direct_n : P.nx;
-  if (!direct && (!runs[inst].active || !psi || nx <= 0)) return;
-  const int comp = blockIdx.y;
-  const int tile = xcd_tile(blockIdx.x, nx / W);
   const int c = threadIdx.x % W;
   const int t = threadIdx.x / W;
   const int jx = tile * W + c;
@@ -264,7 +298,14 @@ __global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __re
   // then the kinetic tables and twiddles (staged into LDS).
   constexpr int kBlock = W * T;
   C a[E];
-  if (stream) {
+  // CM & CM_LOAD / CM_STORE: device-coherent state reads / writes; element
+  // (comp, jy, jx) at byte ((comp N + jy) nx + jx) sizeof(C)
+  const CoherentBuf cb(psi, CM ? (uint32_t)(2 * (size_t)N * nx * sizeof(C)) : 0u);
+  const uint32_t cbase = (uint32_t)(((size_t)comp * N * nx + jx) * sizeof(C));
+  if constexpr ((CM & CM_LOAD) != 0) {
+#pragma unroll
+    for (int j = 0; j < E; ++j) a[j] = cb.load<C>(cbase + (uint32_t)((t + j * T) * nx * sizeof(C)));
+  } else if (stream) {
 #pragma unroll
     for (int j = 0; j < E; ++j) a[j] = ld_stream(col + (size_t)(t + j * T) * nx);
   } else {
@@ -285,14 +326,9 @@ __global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __re
   sp.load(td ? tws : reinterpret_cast<const C*>(P.ky[d_prev][comp]));
   sc_.load(td ? tws : reinterpret_cast<const C*>(P.ky[d_cur][comp]));
   stw.load(tws);
-  if (direct) {
-    // The prologue loads stay ahead of the flag test (an inactive slot's
-    // state and tables are still allocated: the host passes direct_psi only
-    // for a loaded slot).
-    asm volatile("" ::: "memory");
-    if (!runs[inst].active) return;
-  }
-  const bool kspace = runs[inst].form == FORM_KSPACE;
+  const int gk = gate();
+  if (gk < 0) return;
+  const bool kspace = gk != 0;
   const C* kyp = sp.store(sm + W * N + N);
   const C* kyc = sc_.store(sm + W * N + 2 * N);
   const C* tw = stw.store(sm + W * N);
@@ -315,7 +351,10 @@ __global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __re
       col_kinetic<R, E, T, true, false>(a, t, nx, jx, td, hp, hc, kxp, kxc, kyp, kyc, kspace, mid, s_acc, n_acc);
   }
   if (mode == COL_MID) fft_run<N, E, true, V>(a, t, tw, lds);
-  if (stream) {
+  if constexpr ((CM & CM_STORE) != 0) {
+#pragma unroll
+    for (int j = 0; j < E; ++j) cb.store<C>(cbase + (uint32_t)((t + j * T) * nx * sizeof(C)), a[j]);
+  } else if (stream) {
 #pragma unroll
     for (int j = 0; j < E; ++j) st_stream(col + (size_t)(t + j * T) * nx, a[j]);
   } else {
@@ -330,9 +369,48 @@ __global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __re
     block_sum<2>(v, smem);
   if (threadIdx.x == 0) {
     const int nt = nx / W;
-    P.part[PART_S * 2 * nt + comp * nt + tile] = v[0];
-    P.part[PART_NRM * 2 * nt + comp * nt + tile] = v[1];
+    double* ps = P.part + PART_S * 2 * nt + comp * nt + tile;
+    double* pn = P.part + PART_NRM * 2 * nt + comp * nt + tile;
+    if constexpr ((CM & CM_STORE) != 0) {
+      st_coherent(ps, v[0]);
+      st_coherent(pn, v[1]);
+    } else {
+      *ps = v[0];
+      *pn = v[1];
+    }
   }
+}
+
+template <typename R, int N, int E, int W, bool DENSE>
+__global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __restrict__ insts,
+                                                             RunDev* __restrict__ runs, void* direct_psi,
+                                                             int direct_n, const typename Cx<R>::T* __restrict__ tws,
+                                                             int mode, int d_prev, int d_cur, int stream) {
+  typedef typename Cx<R>::T C;
+  extern __shared__ double smem[];
+  const int inst = blockIdx.z;
+  const InstDev& P = insts[inst];
+  // A single-slot plan passes its state's address and row length as kernel
+  // arguments (direct_psi): the state loads are issued at once and the
+  // active flag is checked after them. Otherwise the address and length are
+  // read together with the active flag, so the state loads wait for one
+  // round of scalar loads, not two.
+  const bool direct = direct_psi != nullptr;
+  C* const psi = reinterpret_cast<C*>(direct ? direct_psi : P.psi);
+  const int nx = direct ? direct_n : P.nx;
+  if (!direct && (!runs[inst].active || !psi || nx <= 0)) return;
+  auto gate = [&]() -> int {
+    if (direct) {
+      // The prologue loads stay ahead of the flag test (an inactive slot's
+      // state and tables are still allocated: the host passes direct_psi
+      // only for a loaded slot).
+      asm volatile("" ::: "memory");
+      if (!runs[inst].active) return -1;
+    }
+    return runs[inst].form == FORM_KSPACE ? 1 : 0;
+  };
+  col_item<R, N, E, W, DENSE, 0>(P, psi, nx, tws, mode, d_prev, d_cur, stream, blockIdx.y,
+                                     xcd_tile(blockIdx.x, nx / W), smem, gate);
 }
 
 // Sum of one partial slot over all column tiles of both components
@@ -477,14 +555,31 @@ __device__ __forceinline__ void pointwise_points(C (&a0)[E], C (&a1)[E], const I
   }
 }
 
+// Population record of one full step from the column pass's S sums
+// (tensor_propagator.py:194): pops[2 (step - 1) + s] for step > 0 within
+// the run's capacity. Every thread of the wave gets the sums; thread 0
+// writes.
+template <bool COH = false>
+__device__ __forceinline__ void record_pops(const InstDev& P, double* pops, int cap, int step, int nt2) {
+  const double* part = P.part;
+  double ps[2] = {wave_sum_strided<COH>(part + PART_S * nt2, nt2 / 2),
+                  wave_sum_strided<COH>(part + PART_S * nt2 + nt2 / 2, nt2 / 2)};
+  if (threadIdx.x == 0 && step > 0 && pops && step - 1 < cap) {
+    const double tot = ps[0] + ps[1];
+    pops[2 * (step - 1) + 0] = P.atom_num * ps[0] / tot;
+    pops[2 * (step - 1) + 1] = P.atom_num * ps[1] / tot;
+  }
+}
+
+// One block of ROWS row pairs (both components) of one instance (the body of
+// row_pass_kernel and of the persistent step kernel). `gate` is called once
+// every global load of the prologue has been issued; false drops the block.
 // GENERAL: some slot has a dense potential or a coupling array; otherwise
 // every slot takes the separable fast path.
-template <typename R, int N, int E, int ROWS, bool GENERAL>
-__global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* __restrict__ insts,
-                                                                RunDev* __restrict__ runs, void* direct_psi,
-                                                                int direct_n,
-                                                                const typename Cx<R>::T* __restrict__ tws,
-                                                                int dur, int first_sub, int col_w, int stream) {
+template <typename R, int N, int E, int ROWS, bool GENERAL, int CM, class Gate>
+__device__ __forceinline__ void row_item(const InstDev& P, typename Cx<R>::T* __restrict__ psi, int ny,
+                                         const typename Cx<R>::T* __restrict__ tws, int dur, int col_w, int stream,
+                                         int blk, double* smem, Gate gate) {
   typedef typename Cx<R>::T C;
   typedef SwzView<sizeof(C)> V;
   constexpr int T = N / E;
@@ -494,21 +589,10 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
   constexpr size_t kData = (size_t)ROWS * 2 * N * sizeof(C);
   constexpr bool kTw = tw_in_lds(kData, N, sizeof(C));
   constexpr bool kDb = row_double_buffer(E, kData, kTw ? (size_t)N * sizeof(C) : 0);
-  extern __shared__ double smem[];
-  const int inst = blockIdx.z;
-  RunDev& run = runs[inst];
-  const InstDev& P = insts[inst];
-  // State address and column length: kernel arguments for a single-slot
-  // plan (loads first, active flag checked after them), else read together
-  // with the active flag (one round of scalar loads before the state loads).
-  const bool direct = direct_psi != nullptr;
-  C* const psi = reinterpret_cast<C*>(direct ? direct_psi : P.psi);
-  const int ny = direct ? direct_n : P.ny;
-  if (!direct && (!run.active || !psi || ny <= 0)) return;
   const int nt2 = 2 * (N / col_w);
   const int row = threadIdx.x / T;
   const int t = threadIdx.x % T;
-  const int y = blockIdx.x * ROWS + row;
+  const int y = blk * ROWS + row;
   C* const sm = reinterpret_cast<C*>(smem);
   C* r0 = psi + (size_t)y * N;
   C* r1 = r0 + (size_t)ny * N;
@@ -519,7 +603,17 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
   // they are stored to LDS) and the norm partials; the kernel then waits for
   // memory once.
   C a0[E], a1[E];
-  if (stream) {
+  // CM: as in col_item; element (comp, y, x) at byte ((comp ny + y) N + x) sizeof(C)
+  const CoherentBuf cb(psi, CM ? (uint32_t)(2 * (size_t)ny * N * sizeof(C)) : 0u);
+  const uint32_t c0 = (uint32_t)(((size_t)y * N + t) * sizeof(C));
+  const uint32_t c1 = c0 + (uint32_t)((size_t)ny * N * sizeof(C));
+  if constexpr ((CM & CM_LOAD) != 0) {
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      a0[j] = cb.load<C>(c0 + (uint32_t)(j * T * sizeof(C)));
+      a1[j] = cb.load<C>(c1 + (uint32_t)(j * T * sizeof(C)));
+    }
+  } else if (stream) {
 #pragma unroll
     for (int j = 0; j < E; ++j) {
       a0[j] = ld_stream(r0 + t + j * T);
@@ -552,12 +646,7 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
   if constexpr (kEarlyTables) load_x_tables<E, T>(tx0, tx1, tex, px0, px1, eth, tws, t);
   TableStage<kTw, N, kBlock, C> stw;
   stw.load(tws);
-  if (direct) {
-    // The prologue loads stay ahead of the flag test (a loaded slot's state
-    // and tables stay allocated when it is inactive).
-    asm volatile("" ::: "memory");
-    if (!run.active) return;
-  }
+  if (!gate()) return;
 
   // Real-space normalization (tensor_tools.py:301-305) from the column
   // pass's k-space sums by Parseval: sum_r |IDFT2 v|^2 = Nx Ny sum_k |v|^2.
@@ -565,26 +654,9 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
   // block: no barrier, deterministic); the loads join the ones above. The
   // twiddles stored here are published by the transform's first barrier
   // (its first stage uses none).
-  const bool pops_block = first_sub && blockIdx.x == 0;
   const double* part = P.part;
-  double nrm[1] = {wave_sum_strided(part + PART_NRM * nt2, nt2)};
+  double nrm[1] = {wave_sum_strided<(CM & CM_LOAD) != 0>(part + PART_NRM * nt2, nt2)};
   const C* tw = stw.store(sm + ROWS * 2 * N);
-  if (pops_block) {
-    double ps[2] = {wave_sum_strided(part + PART_S * nt2, nt2 / 2),
-                    wave_sum_strided(part + PART_S * nt2 + nt2 / 2, nt2 / 2)};
-    if (threadIdx.x == 0) {
-      const int step = run.step;
-      if (step > 0 && run.pops && step - 1 < run.pops_cap) {
-        const double tot = ps[0] + ps[1];
-        run.pops[2 * (step - 1) + 0] = P.atom_num * ps[0] / tot;
-        run.pops[2 * (step - 1) + 1] = P.atom_num * ps[1] / tot;
-      }
-      run.step = step + 1;
-    }
-  }
-  // The state leaves this pass in hybrid form (no other block of this kernel
-  // reads the flag; the next column pass does).
-  if (blockIdx.x == 0 && threadIdx.x == 0) run.form = FORM_HYBRID;
   const double lam2 = P.atom_num / (P.dv_r * (double)N * (double)ny * nrm[0]);
 
   C* const alt0 = sm + ROWS * 2 * N + (kTw ? N : 0) + (size_t)row * 2 * N;
@@ -636,7 +708,13 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
     fft_run2_db<N, E, false, V>(a0, a1, t, tw, lds0, lds1, alt0, alt1);
   else
     fft_run2<N, E, false, V>(a0, a1, t, tw, lds0, lds1);
-  if (stream) {
+  if constexpr ((CM & CM_STORE) != 0) {
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      cb.store<C>(c0 + (uint32_t)(j * T * sizeof(C)), a0[j]);
+      cb.store<C>(c1 + (uint32_t)(j * T * sizeof(C)), a1[j]);
+    }
+  } else if (stream) {
 #pragma unroll
     for (int j = 0; j < E; ++j) {
       st_stream(r0 + t + j * T, a0[j]);
@@ -649,6 +727,48 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
       r1[t + j * T] = a1[j];
     }
   }
+}
+
+template <typename R, int N, int E, int ROWS, bool GENERAL>
+__global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* __restrict__ insts,
+                                                                RunDev* __restrict__ runs, void* direct_psi,
+                                                                int direct_n,
+                                                                const typename Cx<R>::T* __restrict__ tws,
+                                                                int dur, int first_sub, int col_w, int stream) {
+  typedef typename Cx<R>::T C;
+  extern __shared__ double smem[];
+  const int inst = blockIdx.z;
+  RunDev& run = runs[inst];
+  const InstDev& P = insts[inst];
+  // State address and column length: kernel arguments for a single-slot
+  // plan (loads first, active flag checked after them), else read together
+  // with the active flag (one round of scalar loads before the state loads).
+  const bool direct = direct_psi != nullptr;
+  C* const psi = reinterpret_cast<C*>(direct ? direct_psi : P.psi);
+  const int ny = direct ? direct_n : P.ny;
+  if (!direct && (!run.active || !psi || ny <= 0)) return;
+  auto gate = [&]() -> bool {
+    if (direct) {
+      // The prologue loads stay ahead of the flag test (a loaded slot's
+      // state and tables stay allocated when it is inactive).
+      asm volatile("" ::: "memory");
+      if (!run.active) return false;
+    }
+    // Block 0 of the first sub-step records the previous full step's
+    // populations and counts the step; the state leaves this pass in hybrid
+    // form (no other block of this kernel reads the flag; the next column
+    // pass does).
+    if (blockIdx.x == 0) {
+      if (first_sub) {
+        const int step = run.step;
+        record_pops(P, run.pops, run.pops_cap, step, 2 * (N / col_w));
+        if (threadIdx.x == 0) run.step = step + 1;
+      }
+      if (threadIdx.x == 0) run.form = FORM_HYBRID;
+    }
+    return true;
+  };
+  row_item<R, N, E, ROWS, GENERAL, 0>(P, psi, ny, tws, dur, col_w, stream, blockIdx.x, smem, gate);
 }
 
 // --------------------------------------------------------------- normalize

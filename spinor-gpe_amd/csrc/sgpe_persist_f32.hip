@@ -1,0 +1,4 @@
+// complex64 instantiations of the persistent step kernel.
+#include "sgpe_persist.h"
+
+SGPE_DEFINE_PERSIST_REGISTRY(persist_registry_f32, float, 1)

@@ -81,6 +81,11 @@ def _declare(lib):
         'sgpe_plan_get_tuning': (i, [vp, ip]),
         'sgpe_plan_set_streaming': (i, [vp, i, i]),
         'sgpe_plan_get_streaming': (i, [vp, ip]),
+        'sgpe_plan_set_persistent': (i, [vp, i, ctypes.c_longlong]),
+        'sgpe_plan_get_persistent': (i, [vp, ip]),
+        'sgpe_plan_check': (i, [vp]),
+        'sgpe_list_persistent': (i, [i, i, ip, i]),
+        'sgpe_profile_persistent': (i, [vp, i, _dp]),
         'sgpe_single_step': (i, [vp, i]),
         'sgpe_full_steps': (i, [vp, i]),
         'sgpe_graph_steps': (i, [vp, i, i]),

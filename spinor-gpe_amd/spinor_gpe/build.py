@@ -25,7 +25,8 @@ ARCH = os.environ.get('SGPE_ARCH', 'gfx950')
 FLAGS = ['-O3', '-std=c++17', f'--offload-arch={ARCH}', '-fPIC', '-Wno-unused-result',
          '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
          '-mllvm', '-amdgpu-kernarg-preload-count=16']
-SOURCES = ['sgpe_api.hip', 'sgpe_inst_f64.hip', 'sgpe_inst_f32.hip', 'sgpe_unwrap.cpp']
+SOURCES = ['sgpe_api.hip', 'sgpe_inst_f64.hip', 'sgpe_inst_f32.hip', 'sgpe_persist_f64.hip',
+           'sgpe_persist_f32.hip', 'sgpe_unwrap.cpp']
 
 
 def _run(cmd, verbose):

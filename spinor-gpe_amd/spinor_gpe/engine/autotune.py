@@ -109,6 +109,18 @@ def shapes(pass_, precision, n):
     return [(buf[2 * i], buf[2 * i + 1]) for i in range(cnt)]
 
 
+def persistent_shapes(precision, n):
+    """Shapes (row E, rows per block, column E, columns per block) with a
+    compiled persistent step kernel for n x n grids."""
+    prec = nat.precision_of(precision)
+    cnt = nat.lib().sgpe_list_persistent(prec, n, None, 0)
+    if cnt < 0:
+        nat.check(cnt)
+    buf = (ctypes.c_int * (4 * cnt))()
+    nat.lib().sgpe_list_persistent(prec, n, buf, cnt)
+    return [tuple(buf[4 * i:4 * i + 4]) for i in range(cnt)]
+
+
 def synthetic_spec(ny, nx, time_mode='real'):
     """A Raman-coupled Thomas-Fermi-like instance of the given grid (lab frame)."""
     r = 16.0

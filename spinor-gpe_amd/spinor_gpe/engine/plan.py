@@ -209,6 +209,29 @@ class Plan:
         nat.check(nat.lib().sgpe_plan_get_streaming(self.handle, out))
         return tuple(out)
 
+    def set_persistent(self, on, wait_ticks=0):
+        """Run blocks of full steps as one persistent launch where available
+        (see ``sgpe_plan_set_persistent``); ``wait_ticks`` bounds each wait
+        of a workgroup, in 100 MHz ticks (0: the library default)."""
+        self._call('sgpe_plan_set_persistent', int(bool(on)), int(wait_ticks))
+
+    def persistent(self):
+        """(requested, used for the next block, grid size or 0)."""
+        out = (ctypes.c_int * 3)()
+        nat.check(nat.lib().sgpe_plan_get_persistent(self.handle, out))
+        return tuple(out)
+
+    def check(self):
+        """Synchronize; raise if a persistent launch stopped on its wait bound."""
+        self._call('sgpe_plan_check')
+
+    def profile_persistent(self, n):
+        """Milliseconds of ``n`` full steps as one persistent launch."""
+        out = ctypes.c_double()
+        self.ctx.bind_stream()
+        nat.check(nat.lib().sgpe_profile_persistent(self.handle, int(n), ctypes.byref(out)))
+        return out.value
+
     def single_step(self, which):
         self._call('sgpe_single_step', int(which))
 

@@ -12,7 +12,9 @@ state and arithmetic, float64 reductions), ``block_steps`` (full steps per
 captured graph), ``progress`` (tqdm bar, default on as in the reference) and
 ``conv_tol`` (> 0: stop once the GP energy changes by at most this relative
 amount per full step, checked every ``block_steps`` steps; the result then
-holds the steps actually run).
+holds the steps actually run) and ``persistent`` (run each block of steps as
+one launch of the persistent step kernel where one is compiled for the
+grid's shapes).
 """
 
 import numpy as np
@@ -66,7 +68,7 @@ class TensorPropagator:
     # pylint: disable=too-many-instance-attributes,too-many-arguments
     def __init__(self, spin, t_step, n_steps, device='cpu', time='imag',
                  is_sampling=False, n_samples=1, *, precision='c128',
-                 block_steps=64, progress=True, conv_tol=0.0):
+                 block_steps=64, progress=True, conv_tol=0.0, persistent=False):
         """Set up a propagation (tensor_propagator.py:67-149)."""
         self._dev = nat.require_device(device)
         self.n_steps = n_steps
@@ -122,6 +124,8 @@ class TensorPropagator:
         self._plan = Plan(self._dev, precision, spec.ny, spec.nx, 1)
         self._state = self._plan.new_state(spin.psik)
         self._plan.set_slot(0, spec, self._state)
+        if persistent:
+            self._plan.set_persistent(True)
         self._pending = False  # True while the state awaits its closing half-step
 
     # ----------------------------------------- operator grids (on demand)
@@ -264,6 +268,7 @@ class TensorPropagator:
         # single_step() calls would write populations into freed memory.
         self._plan.set_run(0, None, conv_tol=0.0)
         del pops_dev
+        self._plan.check()
 
         psik = ttools.to_numpy(self.psik)
         psi = ttools.to_numpy(ttools.ifft_2d(self.psik, self.space['dr']))

@@ -19,8 +19,16 @@ time steps of dt = 1e-3. The fixture keeps, instead of the 8 MiB state:
 * ``gp``: the GP energy terms of the final state (E_tot, E_kin, E_pot,
   E_int, E_cpl, N, mu).
 
-Usage: ``python tests/golden/make_c1_fixture.py`` (about 45 minutes on 8
-cores); writes ``tests/golden/c1_oracle_20000.npz``.
+With ``--numpy-fft`` the oracle's transforms are NumPy's (pocketfft)
+instead of torch's, everything else unchanged, and the fixture goes to
+``c1_oracle_20000_npfft.npz``: two FP64 restatements of the same path that
+differ only in the rounding of their FFTs. Their distance measures how far
+rounding alone carries this run: the dynamics amplify per-step rounding
+differences (about 1e-13 after 1000 steps, 1e-11 after 10000, 1e-10 after
+20000).
+
+Usage: ``python tests/golden/make_c1_fixture.py [--numpy-fft]`` (about 45
+minutes on 8 cores); writes ``tests/golden/c1_oracle_20000.npz``.
 """
 
 import os
@@ -68,10 +76,31 @@ def c0_ground_oracle():
     return ps0, o
 
 
+def use_numpy_fft():
+    """Swap the oracle's torch FFTs for NumPy's (same conventions)."""
+    import torch  # pylint: disable=import-outside-toplevel
+    from oracle import reference_path as rp  # pylint: disable=import-outside-toplevel
+
+    def fft_2d(psi, delta_r):
+        c = rp.prod(delta_r) / (2 * np.pi)
+        return [torch.from_numpy(np.fft.fftshift(np.fft.fftn(p.numpy()) * c.item())) for p in psi]
+
+    def ifft_2d(psik, delta_r):
+        c = rp.prod(delta_r) / (2 * np.pi)
+        return [torch.from_numpy(np.fft.ifftn(np.fft.ifftshift(pk.numpy())) / c.item()) for pk in psik]
+
+    rp.fft_2d = fft_2d
+    rp.ifft_2d = ifft_2d
+
+
 def main():
     from oracle import energy as oe  # pylint: disable=import-outside-toplevel
     from oracle.reference_path import OraclePropagator  # pylint: disable=import-outside-toplevel
     from spinor_gpe.engine import workloads as wl  # pylint: disable=import-outside-toplevel
+    out = OUT
+    if '--numpy-fft' in sys.argv:
+        use_numpy_fft()
+        out = OUT.replace('.npz', '_npfft.npz')
     os.environ.setdefault('SPINOR_GPE_DATA', tempfile.mkdtemp(prefix='sgpe-golden-'))
     t0 = time.time()
     ps0, o0 = c0_ground_oracle()
@@ -99,12 +128,12 @@ def main():
     g = oe.gp_energy(final, ps.space['dr'], ps.kin_eng_spin, ps.pot_eng_spin, ps.g_sc,
                      ps.coupling, theta)
     gp = np.array([g[k] for k in ('E_tot', 'E_kin', 'E_pot', 'E_int', 'E_cpl', 'N', 'mu')])
-    np.savez_compressed(OUT, c0_sub=psik0[:, ::SUB_C0, ::SUB_C0], init_sub=init[:, ::SUB_CK, ::SUB_CK],
+    np.savez_compressed(out, c0_sub=psik0[:, ::SUB_C0, ::SUB_C0], init_sub=init[:, ::SUB_CK, ::SUB_CK],
                         init_proj=project(init, r), ck_steps=np.array(CK_STEPS), ck_sub=np.stack(ck_sub),
                         ck_proj=np.stack(ck_proj), ck_norm=np.array(ck_norm), pops=pops,
                         pops_every=POPS_EVERY, gp=gp, proj_seed=PROJ_SEED, proj_k=PROJ_K,
                         sub_c0=SUB_C0, sub_ck=SUB_CK)
-    print(f"wrote {OUT} ({time.time() - t0:.0f} s)", flush=True)
+    print(f"wrote {out} ({time.time() - t0:.0f} s)", flush=True)
 
 
 if __name__ == '__main__':
