@@ -115,6 +115,18 @@ int sgpe_fft2(sgpe_ctx* ctx, int precision, int ny, int nx, int n_arrays, void* 
 int sgpe_norm(sgpe_ctx* ctx, int precision, int ny, int nx, void* psi0, void* psi1, double vol_elem,
               double atom_num, void* dens0, void* dens1, double* nf_out);
 
+/* sgpe_norm without any allocation. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments and results as sgpe_norm; its reduction partials and norm factor
+ * live in a static 16.6 KiB device array of the library, one per device (the
+ * partials of any grid fit: the reduction strides over the grid with at most
+ * 1024 blocks), so it calls no hipMalloc/hipFree. Every call on a device
+ * shares that array: calls that overlap on one device (different streams or
+ * threads) race and must be serialized by the caller. */
+int sgpe_norm_noalloc(sgpe_ctx* ctx, int precision, int ny, int nx, void* psi0, void* psi1, double vol_elem,
+                      double atom_num, void* dens0, void* dens1, double* nf_out);
+
 /* calc_pops (tensor_tools.py:466-484): out2[s] = vol_elem * sum |psi_s|^2,
  * written to HOST memory (synchronizes the stream). */
 int sgpe_pops(sgpe_ctx* ctx, int precision, int ny, int nx, const void* psi0, const void* psi1, double vol_elem,

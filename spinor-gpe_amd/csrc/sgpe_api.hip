@@ -9,6 +9,7 @@
 #include <cstring>
 #include <exception>
 #include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -158,6 +159,35 @@ static const PEntry* find_persist(int prec, int n, int re, int rr, int ce, int c
   return nullptr;
 }
 
+// Scratch of sgpe_norm_noalloc, one copy per device for the life of the
+// process (16.6 KiB): the reduction partials (two per block, at most
+// kNormBlocks blocks: the kernels stride over larger arrays, so the
+// partials of any grid fit) and the norm factor. Host code never uses the
+// symbol itself, only the device address hipGetSymbolAddress returns for
+// each device (scratch_addr).
+static constexpr int kNormBlocks = 1024;
+static constexpr int kNormScratchDoubles = 2 * kNormBlocks + 32;
+static __device__ __align__(256) double scratch[kNormScratchDoubles];
+
+// Device address of `scratch` on `device`, looked up once per device (with
+// that device current) and kept for the process.
+static int scratch_addr(int device, double** out) {
+  static std::mutex mu;
+  static std::map<int, double*> addr;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = addr.find(device);
+  if (it == addr.end()) {
+    HIP_TRY(hipSetDevice(device));
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(SGPE_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    it = addr.emplace(device, (double*)ptr).first;
+  }
+  *out = it->second;
+  return SGPE_OK;
+}
+
 // ----------------------------------------------------------------- context
 
 struct sgpe_ctx {
@@ -169,6 +199,7 @@ struct sgpe_ctx {
   double* red = nullptr;             // reduction scratch for primitives
   size_t red_count = 0;
   double* red_host = nullptr;        // pinned
+  double* scratch_dev = nullptr;     // this device's address of `scratch` (sgpe_norm_noalloc)
 };
 
 // Stage-major twiddle table of an n-point transform done with e elements per
@@ -793,6 +824,13 @@ int sgpe_ctx_create(int device, void* stream, sgpe_ctx** out) {
       delete c;
       return fail(SGPE_E_HIP, "pinned buffer allocation failed");
     }
+    const int rs = scratch_addr(device, &c->scratch_dev);
+    if (rs) {
+      (void)hipHostFree(c->red_host);
+      (void)hipStreamDestroy(c->cap_stream);
+      delete c;
+      return rs;
+    }
     *out = c;
     return SGPE_OK;
   });
@@ -864,7 +902,36 @@ int sgpe_fft2(sgpe_ctx* ctx, int precision, int ny, int nx, int n_arrays, void* 
   });
 }
 
-static int reduce_blocks(size_t n) { return (int)std::min<size_t>((n + 255) / 256, 1024); }
+static int reduce_blocks(size_t n) { return (int)std::min<size_t>((n + 255) / 256, kNormBlocks); }
+
+// norm() on the context's stream with the partials and norm factor in
+// `part` (2 nb + 1 doubles, nb = reduce_blocks(ny nx)).
+static int launch_norm(sgpe_ctx* ctx, int precision, int ny, int nx, void* psi0, void* psi1, double vol_elem,
+                       double atom_num, void* dens0, void* dens1, double* nf_out, double* part) {
+  const size_t n = (size_t)ny * nx;
+  const int nb = reduce_blocks(n);
+  double* nf_dev = part + 2 * nb;
+  if (precision == SGPE_C128) {
+    hipLaunchKernelGGL(density_sums_kernel<double>, dim3(nb), dim3(256), 0, ctx->stream, (const double2*)psi0,
+                       (const double2*)psi1, n, part);
+    hipLaunchKernelGGL(norm_apply_kernel<double>, dim3(nb), dim3(256), 0, ctx->stream, (double2*)psi0,
+                       (double2*)psi1, n, (const double*)part, nb, vol_elem, atom_num, (double*)dens0,
+                       (double*)dens1, nf_dev);
+  } else {
+    hipLaunchKernelGGL(density_sums_kernel<float>, dim3(nb), dim3(256), 0, ctx->stream, (const float2*)psi0,
+                       (const float2*)psi1, n, part);
+    hipLaunchKernelGGL(norm_apply_kernel<float>, dim3(nb), dim3(256), 0, ctx->stream, (float2*)psi0,
+                       (float2*)psi1, n, (const double*)part, nb, vol_elem, atom_num, (float*)dens0,
+                       (float*)dens1, nf_dev);
+  }
+  HIP_TRY(hipGetLastError());
+  if (nf_out) {
+    HIP_TRY(hipMemcpyAsync(ctx->red_host, nf_dev, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    *nf_out = ctx->red_host[0];
+  }
+  return SGPE_OK;
+}
 
 int sgpe_norm(sgpe_ctx* ctx, int precision, int ny, int nx, void* psi0, void* psi1, double vol_elem,
               double atom_num, void* dens0, void* dens1, double* nf_out) {
@@ -872,31 +939,25 @@ int sgpe_norm(sgpe_ctx* ctx, int precision, int ny, int nx, void* psi0, void* ps
     if (!ctx || !psi0 || !psi1 || ny <= 0 || nx <= 0) return fail(SGPE_E_ARG, "bad arguments");
     if (precision != SGPE_C128 && precision != SGPE_C64) return fail(SGPE_E_ARG, "bad precision");
     HIP_TRY(hipSetDevice(ctx->device));
-    const size_t n = (size_t)ny * nx;
-    const int nb = reduce_blocks(n);
+    const int nb = reduce_blocks((size_t)ny * nx);
     int rc = ctx_reduction(ctx, 2 * (size_t)nb + 1);
     if (rc) return rc;
-    double* nf_dev = ctx->red + 2 * nb;
-    if (precision == SGPE_C128) {
-      hipLaunchKernelGGL(density_sums_kernel<double>, dim3(nb), dim3(256), 0, ctx->stream, (const double2*)psi0,
-                         (const double2*)psi1, n, ctx->red);
-      hipLaunchKernelGGL(norm_apply_kernel<double>, dim3(nb), dim3(256), 0, ctx->stream, (double2*)psi0,
-                         (double2*)psi1, n, (const double*)ctx->red, nb, vol_elem, atom_num, (double*)dens0,
-                         (double*)dens1, nf_dev);
-    } else {
-      hipLaunchKernelGGL(density_sums_kernel<float>, dim3(nb), dim3(256), 0, ctx->stream, (const float2*)psi0,
-                         (const float2*)psi1, n, ctx->red);
-      hipLaunchKernelGGL(norm_apply_kernel<float>, dim3(nb), dim3(256), 0, ctx->stream, (float2*)psi0,
-                         (float2*)psi1, n, (const double*)ctx->red, nb, vol_elem, atom_num, (float*)dens0,
-                         (float*)dens1, nf_dev);
-    }
-    HIP_TRY(hipGetLastError());
-    if (nf_out) {
-      HIP_TRY(hipMemcpyAsync(ctx->red_host, nf_dev, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-      HIP_TRY(hipStreamSynchronize(ctx->stream));
-      *nf_out = ctx->red_host[0];
-    }
-    return SGPE_OK;
+    return launch_norm(ctx, precision, ny, nx, psi0, psi1, vol_elem, atom_num, dens0, dens1, nf_out, ctx->red);
+  });
+}
+
+int sgpe_norm_noalloc(sgpe_ctx* ctx, int precision, int ny, int nx, void* psi0, void* psi1, double vol_elem,
+                      double atom_num, void* dens0, void* dens1, double* nf_out) {
+  return guarded([&]() -> int {
+    if (!ctx || !psi0 || !psi1 || ny <= 0 || nx <= 0) return fail(SGPE_E_ARG, "bad arguments");
+    if (precision != SGPE_C128 && precision != SGPE_C64) return fail(SGPE_E_ARG, "bad precision");
+    const int nb = reduce_blocks((size_t)ny * nx);
+    if (2 * (size_t)nb + 1 > (size_t)kNormScratchDoubles)
+      return fail(SGPE_E_ARG, "sgpe_norm_noalloc: the reduction needs more than the 16.6 KiB scratch");
+    if (!ctx->scratch_dev) return fail(SGPE_E_HIP, "sgpe_norm_noalloc: no scratch address for this device");
+    HIP_TRY(hipSetDevice(ctx->device));
+    return launch_norm(ctx, precision, ny, nx, psi0, psi1, vol_elem, atom_num, dens0, dens1, nf_out,
+                       ctx->scratch_dev);
   });
 }
 

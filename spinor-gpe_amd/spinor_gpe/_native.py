@@ -68,6 +68,7 @@ def _declare(lib):
         'sgpe_ctx_destroy': (None, [vp]),
         'sgpe_fft2': (i, [vp, i, i, i, i, ctypes.POINTER(vp), ctypes.POINTER(vp), i, d, d]),
         'sgpe_norm': (i, [vp, i, i, i, vp, vp, d, d, vp, vp, _dp]),
+        'sgpe_norm_noalloc': (i, [vp, i, i, i, vp, vp, d, d, vp, vp, _dp]),
         'sgpe_pops': (i, [vp, i, i, i, vp, vp, d, _dp]),
         'sgpe_plan_create': (i, [vp, i, i, i, i, ctypes.POINTER(vp)]),
         'sgpe_plan_destroy': (None, [vp]),
