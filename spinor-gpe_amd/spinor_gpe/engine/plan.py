@@ -242,13 +242,14 @@ class Plan:
         self._call('sgpe_graph_steps', int(steps_per_graph), int(n_graphs))
 
     def run(self, n, block=64):
-        """``n`` full steps: whole captured blocks, then eager steps."""
+        """``n`` full steps as captured graphs: ``n // block`` launches of a
+        ``block``-step graph, then one graph of the remaining steps."""
         n = int(n)
         nb, rem = divmod(n, block)
         if nb:
             self.graph_steps(block, nb)
         if rem:
-            self.full_steps(rem)
+            self.graph_steps(rem, 1)
 
     def profile_steps(self, n):
         """Run ``n`` full steps eagerly with a hipEvent after every kernel.

@@ -242,13 +242,18 @@ def request_stream(n, rate, seed=3, grids=((128, 0.3), (256, 0.4), (512, 0.2), (
 
 
 def summarize(outcomes, wall):
-    """Throughput and latency figures of a served run."""
+    """Throughput and latency figures of a served run. ``solves_per_s``
+    counts the results whose ground-state phase met the stop test;
+    ``results_per_s`` counts every result, the unconverged ones (stopped at
+    the step cap) included; latencies are over all results."""
     lat = np.array([o.latency for o in outcomes])
     wait = np.array([o.admitted - o.arrival for o in outcomes])
+    conv = int(sum(o.converged for o in outcomes))
     return {'requests': len(outcomes), 'wall_s': round(wall, 3),
-            'solves_per_s': round(len(outcomes) / wall, 3) if wall > 0 else None,
+            'converged': conv, 'unconverged': len(outcomes) - conv,
+            'solves_per_s': round(conv / wall, 3) if wall > 0 else None,
+            'results_per_s': round(len(outcomes) / wall, 3) if wall > 0 else None,
             'latency_p50_s': round(float(np.percentile(lat, 50)), 4),
             'latency_p99_s': round(float(np.percentile(lat, 99)), 4),
             'queue_wait_p50_s': round(float(np.percentile(wait, 50)), 4),
-            'queue_wait_p99_s': round(float(np.percentile(wait, 99)), 4),
-            'converged': int(sum(o.converged for o in outcomes))}
+            'queue_wait_p99_s': round(float(np.percentile(wait, 99)), 4)}
