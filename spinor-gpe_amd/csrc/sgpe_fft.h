@@ -200,26 +200,41 @@ __device__ __forceinline__ void fft_stage(C (&a)[E], int t, const C* tw, C* lds)
   }
 }
 
+// Orders the LDS writes of one exchange before its reads, and those reads
+// before the next exchange's writes. WAVE: every transform lies within one
+// wave (its threads are consecutive lanes of one wave), whose LDS
+// instructions execute in order, so waiting for the wave's own LDS traffic
+// (and keeping the compiler from moving LDS accesses across) suffices;
+// otherwise the whole workgroup synchronizes.
+template <bool WAVE> __device__ __forceinline__ void exchange_sync() {
+  if constexpr (WAVE)
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  else
+    __syncthreads();
+}
+
 // Full transform: stages of radix E while N/L >= E, then the remainder radix.
-// Every thread of the block must call this (it contains barriers).
-template <int N, int E, bool INV, class V, int L = 1, typename C>
+// Every thread of the block must call this (it contains barriers unless
+// WAVE: see exchange_sync; tables staged in LDS by other waves must then be
+// published by a barrier before the call).
+template <int N, int E, bool INV, class V, bool WAVE = false, int L = 1, typename C>
 __device__ __forceinline__ void fft_run(C (&a)[E], int t, const C* tw, C* lds) {
   constexpr int R = (N / L >= E) ? E : N / L;
   constexpr bool LAST = (L * R == N);
   constexpr int T = N / E;
   fft_stage<N, E, R, L, INV, LAST, V>(a, t, tw, lds);
   if constexpr (!LAST) {
-    __syncthreads();
+    exchange_sync<WAVE>();
 #pragma unroll
     for (int j = 0; j < E; ++j) a[j] = lds[V::at(t + j * T)];
-    __syncthreads();
-    fft_run<N, E, INV, V, L * R>(a, t, tw, lds);
+    exchange_sync<WAVE>();
+    fft_run<N, E, INV, V, WAVE, L * R>(a, t, tw, lds);
   }
 }
 
 // Two independent transforms (e.g. both spinor components of one row) that
 // share barriers: lds0/lds1 are disjoint views.
-template <int N, int E, bool INV, class V, int L = 1, typename C>
+template <int N, int E, bool INV, class V, bool WAVE = false, int L = 1, typename C>
 __device__ __forceinline__ void fft_run2(C (&a)[E], C (&b)[E], int t, const C* tw, C* lds0, C* lds1) {
   constexpr int R = (N / L >= E) ? E : N / L;
   constexpr bool LAST = (L * R == N);
@@ -227,14 +242,14 @@ __device__ __forceinline__ void fft_run2(C (&a)[E], C (&b)[E], int t, const C* t
   fft_stage<N, E, R, L, INV, LAST, V>(a, t, tw, lds0);
   fft_stage<N, E, R, L, INV, LAST, V>(b, t, tw, lds1);
   if constexpr (!LAST) {
-    __syncthreads();
+    exchange_sync<WAVE>();
 #pragma unroll
     for (int j = 0; j < E; ++j) {
       a[j] = lds0[V::at(t + j * T)];
       b[j] = lds1[V::at(t + j * T)];
     }
-    __syncthreads();
-    fft_run2<N, E, INV, V, L * R>(a, b, t, tw, lds0, lds1);
+    exchange_sync<WAVE>();
+    fft_run2<N, E, INV, V, WAVE, L * R>(a, b, t, tw, lds0, lds1);
   }
 }
 
@@ -243,7 +258,7 @@ __device__ __forceinline__ void fft_run2(C (&a)[E], C (&b)[E], int t, const C* t
 // (l0, l1) or (m0, m1) alternately, so a buffer is rewritten only after the
 // next exchange's barrier, which every thread reaches after its reads of
 // that buffer: one barrier per exchange instead of two.
-template <int N, int E, bool INV, class V, int L = 1, typename C>
+template <int N, int E, bool INV, class V, bool WAVE = false, int L = 1, typename C>
 __device__ __forceinline__ void fft_run2_db(C (&a)[E], C (&b)[E], int t, const C* tw, C* l0, C* l1, C* m0, C* m1) {
   constexpr int R = (N / L >= E) ? E : N / L;
   constexpr bool LAST = (L * R == N);
@@ -251,13 +266,13 @@ __device__ __forceinline__ void fft_run2_db(C (&a)[E], C (&b)[E], int t, const C
   fft_stage<N, E, R, L, INV, LAST, V>(a, t, tw, l0);
   fft_stage<N, E, R, L, INV, LAST, V>(b, t, tw, l1);
   if constexpr (!LAST) {
-    __syncthreads();
+    exchange_sync<WAVE>();
 #pragma unroll
     for (int j = 0; j < E; ++j) {
       a[j] = l0[V::at(t + j * T)];
       b[j] = l1[V::at(t + j * T)];
     }
-    fft_run2_db<N, E, INV, V, L * R>(a, b, t, tw, m0, m1, l0, l1);
+    fft_run2_db<N, E, INV, V, WAVE, L * R>(a, b, t, tw, m0, m1, l0, l1);
   }
 }
 

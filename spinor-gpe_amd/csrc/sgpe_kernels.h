@@ -287,6 +287,9 @@ __device__ __forceinline__ void col_item(const InstDev& P, typename Cx<R>::T* __
   constexpr bool kKy = kTw && kData + 3 * (size_t)N * sizeof(C) <= kLdsBytes;
   constexpr size_t kBase = kData + (kTw ? (size_t)N * sizeof(C) : 0) + (kKy ? 2 * (size_t)N * sizeof(C) : 0);
   constexpr bool kSumSep = col_sum_separate(kBase);
+  // The W*T threads of a tile interleave its W columns, so a column's
+  // transform lies within one wave when the whole tile does.
+  constexpr bool kWave = W * T <= 64;
   const int c = threadIdx.x % W;
   const int t = threadIdx.x / W;
   const int jx = tile * W + c;
@@ -333,10 +336,12 @@ __device__ __forceinline__ void col_item(const InstDev& P, typename Cx<R>::T* __
   const C* kyc = sc_.store(sm + W * N + 2 * N);
   const C* tw = stw.store(sm + W * N);
 
-  if (!kspace)
-    fft_run<N, E, false, V>(a, t, tw, lds);
-  else
+  if (!kspace) {
+    if constexpr (kWave && (kTw || kKy)) __syncthreads();  // publishes the staged tables
+    fft_run<N, E, false, V, kWave>(a, t, tw, lds);
+  } else {
     __syncthreads();  // publishes the staged tables
+  }
 
   double s_acc = 0.0, n_acc = 0.0;
   const R hp = (R)(0.5 * P.dt_sub[d_prev]);
@@ -350,7 +355,7 @@ __device__ __forceinline__ void col_item(const InstDev& P, typename Cx<R>::T* __
     else
       col_kinetic<R, E, T, true, false>(a, t, nx, jx, td, hp, hc, kxp, kxc, kyp, kyc, kspace, mid, s_acc, n_acc);
   }
-  if (mode == COL_MID) fft_run<N, E, true, V>(a, t, tw, lds);
+  if (mode == COL_MID) fft_run<N, E, true, V, kWave>(a, t, tw, lds);
   if constexpr ((CM & CM_STORE) != 0) {
 #pragma unroll
     for (int j = 0; j < E; ++j) cb.store<C>(cbase + (uint32_t)((t + j * T) * nx * sizeof(C)), a[j]);
@@ -589,6 +594,9 @@ __device__ __forceinline__ void row_item(const InstDev& P, typename Cx<R>::T* __
   constexpr size_t kData = (size_t)ROWS * 2 * N * sizeof(C);
   constexpr bool kTw = tw_in_lds(kData, N, sizeof(C));
   constexpr bool kDb = row_double_buffer(E, kData, kTw ? (size_t)N * sizeof(C) : 0);
+  // A row's T threads are consecutive lanes: with T <= 64 each row's
+  // transforms lie within one wave.
+  constexpr bool kWave = T <= 64;
   const int nt2 = 2 * (N / col_w);
   const int row = threadIdx.x / T;
   const int t = threadIdx.x % T;
@@ -657,14 +665,15 @@ __device__ __forceinline__ void row_item(const InstDev& P, typename Cx<R>::T* __
   const double* part = P.part;
   double nrm[1] = {wave_sum_strided<(CM & CM_LOAD) != 0>(part + PART_NRM * nt2, nt2)};
   const C* tw = stw.store(sm + ROWS * 2 * N);
+  if constexpr (kWave && kTw) __syncthreads();  // publishes the twiddles staged by every wave
   const double lam2 = P.atom_num / (P.dv_r * (double)N * (double)ny * nrm[0]);
 
   C* const alt0 = sm + ROWS * 2 * N + (kTw ? N : 0) + (size_t)row * 2 * N;
   C* const alt1 = alt0 + N;
   if constexpr (kDb)
-    fft_run2_db<N, E, true, V>(a0, a1, t, tw, lds0, lds1, alt0, alt1);
+    fft_run2_db<N, E, true, V, kWave>(a0, a1, t, tw, lds0, lds1, alt0, alt1);
   else
-    fft_run2<N, E, true, V>(a0, a1, t, tw, lds0, lds1);
+    fft_run2<N, E, true, V, kWave>(a0, a1, t, tw, lds0, lds1);
 
   // The x-table values of this thread's points (an absent potential table
   // is stood in for by the global twiddle table: N entries, unused). Radix
@@ -703,11 +712,11 @@ __device__ __forceinline__ void row_item(const InstDev& P, typename Cx<R>::T* __
   // The forward transform starts on the buffer pair the inverse one read
   // last but one, so no barrier is needed between them.
   if constexpr (kDb && fft_exchanges<N, E>() % 2)
-    fft_run2_db<N, E, false, V>(a0, a1, t, tw, alt0, alt1, lds0, lds1);
+    fft_run2_db<N, E, false, V, kWave>(a0, a1, t, tw, alt0, alt1, lds0, lds1);
   else if constexpr (kDb)
-    fft_run2_db<N, E, false, V>(a0, a1, t, tw, lds0, lds1, alt0, alt1);
+    fft_run2_db<N, E, false, V, kWave>(a0, a1, t, tw, lds0, lds1, alt0, alt1);
   else
-    fft_run2<N, E, false, V>(a0, a1, t, tw, lds0, lds1);
+    fft_run2<N, E, false, V, kWave>(a0, a1, t, tw, lds0, lds1);
   if constexpr ((CM & CM_STORE) != 0) {
 #pragma unroll
     for (int j = 0; j < E; ++j) {
