@@ -313,7 +313,7 @@ struct sgpe_plan {
   bool pending = false;    // a sub-step was issued since the last sync (active states are hybrid)
   std::map<int, hipGraphExec_t> graphs;
   std::map<int, hipGraph_t> graph_defs;
-  void* scratch = nullptr;  // [nslots][2][ny][nx] for energy evaluation
+  bool energy_ready = false;  // energy kernels selected, their LDS set
   const KEntry* ecol = nullptr;
   const KEntry* erow = nullptr;
   int e_col_e = 8, e_col_w = 4, e_row_e = 8, e_row_rows = 1;
@@ -500,30 +500,24 @@ struct sgpe_group {
   std::vector<unsigned> epochs;  // plan epochs the cached graphs were captured at
 };
 
-// Energy work buffer and kernel shapes of a plan (allocates once; must not
-// run during graph capture).
+// Energy kernel shapes of a plan (once; must not run during graph capture).
+// The evaluation works in place on the states: no work buffer.
 static int plan_energy_prepare(sgpe_plan* p) {
-  if (p->scratch) return SGPE_OK;
-  hipStream_t s = p->ctx->stream;
+  if (p->energy_ready) return SGPE_OK;
   const size_t cs = csize(p->prec);
-  const size_t per = (size_t)2 * p->ny * p->nx * cs;
   int rc;
   default_col(p->prec, p->ny, &p->e_col_e, &p->e_col_w);
   default_row(p->prec, p->nx, &p->e_row_e, &p->e_row_rows);
   p->ecol = find_entry(K_ECOL, p->prec, p->ny, p->e_col_e, p->e_col_w);
   p->erow = find_entry(K_EROW, p->prec, p->nx, p->e_row_e, p->e_row_rows);
   if (!p->ecol || !p->erow) return fail(SGPE_E_UNSUPPORTED, "energy kernels not compiled for this grid");
-  if ((rc = set_lds_attr(p->ecol->kernel, lds_bytes((size_t)p->e_col_w * p->ny * cs, p->ny, cs)))) return rc;
+  const size_t cl = lds_bytes((size_t)p->e_col_w * p->ny * cs, p->ny, cs);
+  if ((rc = set_lds_attr(p->ecol->kernel, cl))) return rc;
+  if ((rc = set_lds_attr(p->ecol->kernel_g, cl))) return rc;
   if ((rc = set_lds_attr(p->erow->kernel, lds_bytes((size_t)p->e_row_rows * 2 * p->nx * cs, p->nx, cs)))) return rc;
   if ((rc = ctx_twiddles(p->ctx, p->prec, p->ny, p->e_col_e, &p->etw_y))) return rc;
   if ((rc = ctx_twiddles(p->ctx, p->prec, p->nx, p->e_row_e, &p->etw_x))) return rc;
-  if (dev_alloc(&p->scratch, per * p->nslots) != hipSuccess) {
-    p->scratch = nullptr;
-    return fail(SGPE_E_NOMEM, "energy work buffer allocation failed");
-  }
-  for (int i = 0; i < p->nslots; ++i) p->h_inst[i].scratch = (char*)p->scratch + i * per;
-  HIP_TRY(hipMemcpyAsync(p->d_inst, p->h_inst.data(), p->nslots * sizeof(InstDev), hipMemcpyHostToDevice, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  p->energy_ready = true;
   return SGPE_OK;
 }
 
@@ -538,6 +532,7 @@ static void issue_energy(sgpe_plan* p, hipStream_t s, double* out) {
   dim3 rg(p->ny / p->e_row_rows, 1, p->nslots), rb(p->e_row_rows * (p->nx / p->e_row_e));
   ((EFn)p->ecol->launch)(cg, cb, cl, s, p->d_inst, p->d_run, p->etw_y);
   ((EFn)p->erow->launch)(rg, rb, rl, s, p->d_inst, p->d_run, p->etw_x);
+  ((EFn)p->ecol->launch_g)(cg, cb, cl, s, p->d_inst, p->d_run, p->etw_y);  // restores psik
   hipLaunchKernelGGL(energy_final_kernel, dim3(p->nslots), dim3(256), 0, s, p->d_inst, p->d_run,
                      p->nx / p->e_col_w, p->ny / p->e_row_rows, out);
 }
@@ -663,7 +658,6 @@ static int plan_load_slot(sgpe_plan* p, int slot, const sgpe_inst_desc* d, void*
     }
   const size_t o_part = st.add((size_t)PART_SLOTS * 2 * nx * sizeof(double));
   const size_t o_epart = st.add((size_t)EPART_SLOTS * (2 * nx + ny) * sizeof(double));
-  h.scratch = p->scratch ? (char*)p->scratch + (size_t)slot * 2 * ny * nx * cs : nullptr;
 
   // Phase tables in float64, rounded once to the plan precision.
   for (int dd = 0; dd < 2; ++dd) {
@@ -1041,7 +1035,6 @@ void sgpe_plan_destroy(sgpe_plan* p) {
   drop_graphs(p);
   for (auto& s : p->slots)
     if (s.buf) (void)hipFree(s.buf);
-  if (p->scratch) (void)hipFree(p->scratch);
   if (p->d_inst) (void)hipFree(p->d_inst);
   if (p->d_run) (void)hipFree(p->d_run);
   if (p->d_sync) (void)hipFree(p->d_sync);

@@ -47,7 +47,6 @@ struct InstDev {
   void* psi;        // [2][ny][nx] complex state
   double* part;     // column-pass partial sums, [PART_SLOTS][ntile]
   double* epart;    // energy partial sums, [EPART_SLOTS][estride]
-  void* scratch;    // [2][ny][nx] work state for energy evaluation (or null)
   int estride;      // entries per energy partial slot
   int ny, nx;
   int imag;         // 1: imaginary time (t_step = -i dt), 0: real time

@@ -848,10 +848,15 @@ static __global__ void config_run_kernel(RunDev* __restrict__ runs, int slot, do
 //   E_pot = dv_r sum_s sum_r V_s n_s, E_int = dv_r/2 sum (g_uu n0^2 + g_dd n1^2 + 2 g_ud n0 n1)
 //   E_cpl = dv_r sum Omega Re(psi0* psi1 e^{-i theta}),  mu = (E_kin + E_pot + E_cpl + 2 E_int)/N
 // with real space from chi = IDFT2(psik): n = |chi|^2 / (Nx Ny c)^2 (the
-// (-1)^(x+y) of the shift cancels in every term).
+// (-1)^(x+y) of the shift cancels in every term). The transforms work in
+// place: the column part leaves IDFT_y psik, the row part sums over
+// IDFT_x IDFT_y psik and writes back DFT_x of it scaled by 1/(Nx Ny), and
+// the restoring column part (FWD) applies DFT_y, which brings back psik up
+// to rounding; no second copy of the state is needed.
 
-// Column part: kinetic and norm sums in k-space, then IDFT_y into scratch.
-template <typename R, int N, int E, int W>
+// Column part: kinetic and norm sums in k-space, then IDFT_y in place; or
+// (FWD) DFT_y in place, the last step of the evaluation.
+template <typename R, int N, int E, int W, bool FWD>
 __global__ __launch_bounds__(W*(N / E)) void energy_col_kernel(const InstDev* __restrict__ insts,
                                                                const RunDev* __restrict__ runs,
                                                                const typename Cx<R>::T* __restrict__ tws) {
@@ -869,13 +874,18 @@ __global__ __launch_bounds__(W*(N / E)) void energy_col_kernel(const InstDev* __
   const int t = threadIdx.x / W;
   const int jx = tile * W + c;
   C* lds = reinterpret_cast<C*>(smem) + c;
-  const C* col = reinterpret_cast<const C*>(P.psi) + (size_t)comp * N * nx + jx;
-  C* out = reinterpret_cast<C*>(P.scratch) + (size_t)comp * N * nx + jx;
+  C* col = reinterpret_cast<C*>(P.psi) + (size_t)comp * N * nx + jx;
   constexpr size_t kData = (size_t)W * N * sizeof(C);
   const C* tw = stage_twiddles<tw_in_lds(kData, N, sizeof(C))>(tws, N, reinterpret_cast<C*>(smem) + W * N);
   C a[E];
 #pragma unroll
   for (int j = 0; j < E; ++j) a[j] = col[(size_t)(t + j * T) * nx];
+  if constexpr (FWD) {
+    fft_run<N, E, false, V>(a, t, tw, lds);
+#pragma unroll
+    for (int j = 0; j < E; ++j) col[(size_t)(t + j * T) * nx] = a[j];
+    return;
+  }
   const double* td = P.t_dense[comp];
   const double tx = td ? 0.0 : P.tx[comp][jx];
   double kin = 0.0, s = 0.0;
@@ -889,7 +899,7 @@ __global__ __launch_bounds__(W*(N / E)) void energy_col_kernel(const InstDev* __
   }
   fft_run<N, E, true, V>(a, t, tw, lds);
 #pragma unroll
-  for (int j = 0; j < E; ++j) out[(size_t)(t + j * T) * nx] = a[j];
+  for (int j = 0; j < E; ++j) col[(size_t)(t + j * T) * nx] = a[j];
   double v[2] = {kin, s};
   block_sum<2>(v, smem);
   if (threadIdx.x == 0) {
@@ -899,7 +909,7 @@ __global__ __launch_bounds__(W*(N / E)) void energy_col_kernel(const InstDev* __
   }
 }
 
-// Row part: IDFT_x of scratch rows, real-space sums (nothing is written back).
+// Row part: IDFT_x of the rows, real-space sums, then DFT_x / (Nx Ny) in place.
 template <typename R, int N, int E, int ROWS>
 __global__ __launch_bounds__(ROWS*(N / E)) void energy_row_kernel(const InstDev* __restrict__ insts,
                                                                   const RunDev* __restrict__ runs,
@@ -915,8 +925,8 @@ __global__ __launch_bounds__(ROWS*(N / E)) void energy_row_kernel(const InstDev*
   const int row = threadIdx.x / T;
   const int t = threadIdx.x % T;
   const int y = blockIdx.x * ROWS + row;
-  const C* r0 = reinterpret_cast<const C*>(P.scratch) + (size_t)y * N;
-  const C* r1 = r0 + (size_t)ny * N;
+  C* r0 = reinterpret_cast<C*>(P.psi) + (size_t)y * N;
+  C* r1 = r0 + (size_t)ny * N;
   C* lds0 = reinterpret_cast<C*>(smem) + (size_t)row * 2 * N;
   C* lds1 = lds0 + N;
   constexpr size_t kData = (size_t)ROWS * 2 * N * sizeof(C);
@@ -954,6 +964,13 @@ __global__ __launch_bounds__(ROWS*(N / E)) void energy_row_kernel(const InstDev*
       }
       cpl += om * cr;
     }
+  }
+  fft_run2<N, E, false, V>(a0, a1, t, tw, lds0, lds1);
+  const R back = (R)(1.0 / ((double)N * (double)ny));
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    r0[t + j * T] = cscale(a0[j], back);
+    r1[t + j * T] = cscale(a1[j], back);
   }
   double v[4] = {pot, inter, cpl, nrm};
   block_sum<4>(v, smem);

@@ -72,9 +72,9 @@ void fcol_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const void* in, void* 
   typedef typename Cx<R>::T C;
   hipLaunchKernelGGL((fft_cols_kernel<R, N, E, W>), g, b, l, s, (const C*)in, (C*)out, nx, inv, (const C*)tw);
 }
-template <typename R, int N, int E, int W>
+template <typename R, int N, int E, int W, bool FWD>
 void ecol_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const InstDev* i, const RunDev* r, const void* tw) {
-  hipLaunchKernelGGL((energy_col_kernel<R, N, E, W>), g, b, l, s, i, r, (const typename Cx<R>::T*)tw);
+  hipLaunchKernelGGL((energy_col_kernel<R, N, E, W, FWD>), g, b, l, s, i, r, (const typename Cx<R>::T*)tw);
 }
 template <typename R, int N, int E, int ROWS>
 void erow_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const InstDev* i, const RunDev* r, const void* tw) {
@@ -107,9 +107,17 @@ void erow_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const InstDev* i, cons
 #define SGPE_FCOL_E(R, PR, N, E, W)                                                                          \
   {PR, N, E, W, (void*)&sgpe::fcol_launch<R, N, E, W>, (const void*)&sgpe::fft_cols_kernel<R, N, E, W>, \
    nullptr, nullptr},
-#define SGPE_ECOL_E(R, PR, N, E, W)                                                                            \
-  {PR, N, E, W, (void*)&sgpe::ecol_launch<R, N, E, W>, (const void*)&sgpe::energy_col_kernel<R, N, E, W>, \
-   nullptr, nullptr},
+// Energy column kernels: the summing inverse pass, and (launch_g/kernel_g)
+// the restoring forward pass.
+#define SGPE_ECOL_E(R, PR, N, E, W)                                                                              \
+  {PR,                                                                                                           \
+   N,                                                                                                            \
+   E,                                                                                                            \
+   W,                                                                                                            \
+   (void*)&sgpe::ecol_launch<R, N, E, W, false>,                                                                 \
+   (const void*)&sgpe::energy_col_kernel<R, N, E, W, false>,                                                     \
+   (void*)&sgpe::ecol_launch<R, N, E, W, true>,                                                                  \
+   (const void*)&sgpe::energy_col_kernel<R, N, E, W, true>},
 #define SGPE_EROW_E(R, PR, N, E, RW)                                                                             \
   {PR, N, E, RW, (void*)&sgpe::erow_launch<R, N, E, RW>, (const void*)&sgpe::energy_row_kernel<R, N, E, RW>, \
    nullptr, nullptr},

@@ -75,7 +75,8 @@ class _Job:
 
 def state_bytes(grid, precision):
     """Device bytes a request holds while admitted: its [2, n, n] state plus
-    its slot tables (1-D phase tables, norm partials, energy partials)."""
+    its slot tables (1-D phase tables, norm partials, energy partials). The
+    energy evaluation works in place on the state, so this is all."""
     b = 16 if nat.precision_of(precision) == nat.C128 else 8
     return 2 * grid * grid * b + 64 * grid * b
 
@@ -85,8 +86,8 @@ class Server:
 
     ``slots``: slots per grid size; ``hbm_budget``: bytes of device memory the
     server may hold for requests (default: 80 % of the free memory when the
-    server starts); each plan's fixed cost (energy work buffer of all its
-    slots) is charged when the plan is created.
+    server starts); a request is charged its state and slot tables while it
+    is admitted.
     """
 
     # pylint: disable=too-many-instance-attributes,too-many-arguments
@@ -120,12 +121,8 @@ class Server:
             if grid not in self.slots:
                 raise ValueError(f"no slots configured for grid {grid}")
             n = self.slots[grid]
-            fixed = n * state_bytes(grid, self.precision)  # energy work buffer
-            if self.used + fixed > self.budget:
-                return None
             self.plans[grid] = Plan(self.device, self.precision, grid, grid, n)
             self.free[grid] = list(range(n))
-            self.used += fixed
         return self.plans[grid]
 
     def _group(self, grids):

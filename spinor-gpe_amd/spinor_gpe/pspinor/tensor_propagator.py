@@ -17,6 +17,8 @@ one launch of the persistent step kernel where one is compiled for the
 grid's shapes).
 """
 
+import warnings
+
 import numpy as np
 import torch
 
@@ -69,8 +71,23 @@ class TensorPropagator:
     def __init__(self, spin, t_step, n_steps, device='cpu', time='imag',
                  is_sampling=False, n_samples=1, *, precision='c128',
                  block_steps=64, progress=True, conv_tol=0.0, persistent=False):
-        """Set up a propagation (tensor_propagator.py:67-149)."""
+        """Set up a propagation (tensor_propagator.py:67-149).
+
+        ``device='cpu'`` (the reference's default) runs the engine on the
+        current HIP GPU, since there is no CPU propagation path, and presents
+        the state and operator grids as host tensors, as the reference's CPU
+        run would; any other device must be a HIP GPU.
+        """
+        self._view_dev = None
+        if torch.device(device).type == 'cpu':
+            warnings.warn("TensorPropagator: the split-step engine runs only on HIP GPUs; "
+                          "device='cpu' runs it on the current GPU and returns host tensors",
+                          RuntimeWarning, stacklevel=2)
+            self._view_dev = torch.device('cpu')
+            device = 'cuda'
         self._dev = nat.require_device(device)
+        if self._view_dev is None:
+            self._view_dev = self._dev
         self.n_steps = n_steps
         self.device = device
         self.paths = spin.paths
@@ -101,7 +118,7 @@ class TensorPropagator:
         keys_space = ['dr', 'dk', 'x_mesh', 'y_mesh', 'dv_r', 'dv_k']
         self._space_host = {k: np.asarray(spin.space[k]) for k in keys_space}
         self._coupling_host = np.asarray(spin.coupling, dtype=np.float64)
-        self.space = _DeviceDict(self._space_host, self._dev)
+        self.space = _DeviceDict(self._space_host, self._view_dev)
         self._device_copies = {}
         # pylint: disable=invalid-name
         self.kL_recoil = spin.kL_recoil
@@ -137,17 +154,17 @@ class TensorPropagator:
     @property
     def kin_eng_spin(self):
         """Kinetic energy grids as device tensors (tensor_propagator.py:114)."""
-        return self._device_copy('kin', lambda: ttools.to_tensor(self._kin_host, dev=self._dev))
+        return self._device_copy('kin', lambda: ttools.to_tensor(self._kin_host, dev=self._view_dev))
 
     @property
     def pot_eng_spin(self):
         """Potential energy grids as device tensors (tensor_propagator.py:115)."""
-        return self._device_copy('pot', lambda: ttools.to_tensor(self._pot_host, dev=self._dev))
+        return self._device_copy('pot', lambda: ttools.to_tensor(self._pot_host, dev=self._view_dev))
 
     @property
     def coupling(self):
         """Coupling grid as a device tensor (tensor_propagator.py:121)."""
-        return self._device_copy('coupling', lambda: ttools.to_tensor(self._coupling_host, dev=self._dev))
+        return self._device_copy('coupling', lambda: ttools.to_tensor(self._coupling_host, dev=self._view_dev))
 
     @property
     def expon(self):
@@ -160,8 +177,12 @@ class TensorPropagator:
     # ------------------------------------------------------------- state
     @property
     def psik(self):
-        """Current normalized k-space state as ``[psik_up, psik_down]`` views."""
+        """Current normalized k-space state as ``[psik_up, psik_down]``: views
+        of the device state, or host copies when the propagator was made with
+        ``device='cpu'``."""
         self._settle()
+        if self._view_dev.type == 'cpu':
+            return [self._state[0].cpu(), self._state[1].cpu()]
         return [self._state[0], self._state[1]]
 
     @psik.setter
@@ -270,8 +291,10 @@ class TensorPropagator:
         del pops_dev
         self._plan.check()
 
-        psik = ttools.to_numpy(self.psik)
-        psi = ttools.to_numpy(ttools.ifft_2d(self.psik, self.space['dr']))
+        self._settle()
+        on_device = [self._state[0], self._state[1]]
+        psik = ttools.to_numpy(on_device)
+        psi = ttools.to_numpy(ttools.ifft_2d(on_device, self._space_host['dr']))
         energy = self.eng_expect(psik, psi=psi)
 
         if self.is_sampling:

@@ -10,6 +10,8 @@
   sub-step, full step.
 * A slot whose tables grow keeps its previous physics when the new device
   allocation fails (NOMEM), and the plan still steps correctly.
+* device='cpu', the reference's default, runs on the GPU and hands back host
+  tensors.
 """
 
 import numpy as np
@@ -90,3 +92,17 @@ def test_failed_table_growth_keeps_slot(gpu):
     o = OraclePropagator(ps, dt, time=time)
     o.run(3)
     assert rel(st.cpu().numpy(), np.stack(o.psik_numpy())) <= 1e-12
+
+
+def test_cpu_device_argument_runs_on_the_gpu(gpu):
+    """The reference's default device='cpu' (pspinor.py:890, 927): the engine
+    runs on the GPU, warns, and presents host tensors."""
+    ps, time, dt = ground(name='cpudev', mesh=(64, 64))
+    oracle = OraclePropagator(ps, dt, time=time)
+    ref_pops = oracle.run(20)
+    with pytest.warns(RuntimeWarning, match='HIP'):
+        res, prop = ps.imaginary(dt, 20, progress=False)
+    assert all(p.device.type == 'cpu' for p in prop.psik)
+    assert prop.space['dr'].device.type == 'cpu'
+    assert rel(np.stack(res.psik), np.stack(oracle.psik_numpy())) <= 1e-12
+    assert np.max(np.abs(res.pops['vals'] - ref_pops)) <= 1e-12 * ps.atom_num

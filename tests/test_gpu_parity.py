@@ -168,8 +168,3 @@ def test_convergence_stop(gpu):
     res2, _ = ps.imaginary(dt, 50, 'cuda', progress=False, block_steps=50)
     assert abs(res2.energy_gp['E_tot'] / res.energy_gp['E_tot'] - 1) < 50 * 1e-9 * 2
 
-
-def test_cpu_device_raises(gpu):
-    ps, time, dt = CASES['ground'](name='cpu')
-    with pytest.raises(ValueError, match='HIP'):
-        ps.imaginary(dt, 2, 'cpu')
