@@ -103,7 +103,7 @@ void sgpe_ctx_destroy(sgpe_ctx* ctx);
  * n_arrays device arrays of shape (ny, nx):
  *   inverse = 0: out = fftshift(fftn(in)) * dx*dy/(2 pi)
  *   inverse = 1: out = ifftn(ifftshift(in)) / (dx*dy/(2 pi))
- * in may equal out. */
+ * in may equal out. Grid lengths as for sgpe_plan_create. */
 int sgpe_fft2(sgpe_ctx* ctx, int precision, int ny, int nx, int n_arrays, void* const* in, void* const* out,
               int inverse, double dx, double dy);
 
@@ -136,7 +136,12 @@ int sgpe_pops(sgpe_ctx* ctx, int precision, int ny, int nx, const void* psi0, co
 
 /* A plan of n_slots instances sharing one grid (ny, nx) and precision.
  * Stands in for TensorPropagator's operator set-up (tensor_propagator.py:
- * 67-149), batched. ny, nx: powers of two in [64, 4096]. Slots start empty
+ * 67-149), batched. ny, nx: even lengths up to 2048 (the reference's
+ * requirement, pspinor.py:331-332), and powers of two up to 4096. Powers of
+ * two from 64 up run on compiled register-FFT passes; other lengths (and a
+ * power-of-two axis no compiled tile of which divides the other axis) on
+ * runtime-length LDS mixed-radix passes, slower but with the same results
+ * to rounding. Other sizes: SGPE_E_UNSUPPORTED. Slots start empty
  * and inactive; a plan can step once at least one slot is set (empty slots
  * are skipped), so a server can fill slots as requests arrive. */
 int sgpe_plan_create(sgpe_ctx* ctx, int precision, int ny, int nx, int n_slots, sgpe_plan** out);
@@ -180,7 +185,8 @@ int sgpe_plan_tune(sgpe_plan* plan, int row_e, int row_rows, int col_e, int col_
  * the number available (>= 0) or a negative error. Feeds the autotuner. */
 int sgpe_list_shapes(int pass, int precision, int n, int* out_pairs, int cap);
 
-/* Current shapes: out4 = {row_e, row_rows, col_e, col_w}. */
+/* Current shapes: out4 = {row_e, row_rows, col_e, col_w}; row_e = 0 (col_e =
+ * 0): that pass runs on the runtime-length kernels. */
 int sgpe_plan_get_tuning(const sgpe_plan* plan, int* out4);
 
 /* Run blocks of full steps (sgpe_full_steps, sgpe_graph_steps, sgpe_group_run)

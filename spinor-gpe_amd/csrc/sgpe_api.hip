@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/sgpe.h"
+#include "sgpe_generic.h"
 #include "sgpe_persist.h"
 #include "sgpe_registry.h"
 
@@ -146,6 +147,67 @@ static int set_lds_attr(const void* kernel, size_t bytes) {
 
 static bool pow2_in_range(int n) { return n >= 64 && n <= 4096 && (n & (n - 1)) == 0; }
 
+// Lengths of the runtime-length kernels (sgpe_generic.h): every even length
+// up to kGenMaxLen. Powers of two from 64 to 4096 have compiled kernels; a
+// power-of-two axis also takes the runtime-length kernels when no compiled
+// tile of its pass divides the other axis.
+static bool gen_length_ok(int n) { return n >= 2 && n <= kGenMaxLen && n % 2 == 0; }
+static bool length_ok(int n) { return pow2_in_range(n) || gen_length_ok(n); }
+static const char* const kLengthMsg = "grid sizes must be even and at most 2048, or powers of two up to 4096";
+
+// Stage radices of an n-point runtime-length transform: 8s, then one 4 or
+// 2, then the odd prime factors in increasing order.
+static GenFft gen_factor(int n) {
+  GenFft f;
+  std::memset(&f, 0, sizeof(f));
+  f.n = n;
+  int m = n;
+  auto push = [&](int r) {
+    f.radix[f.nst++] = r;
+    m /= r;
+  };
+  while (m % 8 == 0) push(8);
+  if (m % 4 == 0) push(4);
+  if (m % 2 == 0) push(2);
+  for (int q = 3; m > 1; q += 2)
+    while (m % q == 0) push(q);
+  return f;
+}
+
+// LDS of the runtime-length kernels: two buffers of the workgroup's
+// transforms plus 256 bytes for block sums.
+static size_t gen_cols_lds(int w, int n, size_t cs) { return 2 * (size_t)w * n * cs + 256; }
+static size_t gen_rows_lds(int rows, int n, size_t cs) { return 2 * (size_t)rows * n * cs + 256; }
+
+// Column tile of the runtime-length column kernels: the widest of 8, 4, 2
+// dividing nx whose buffers stay within 64 KiB, else 1.
+static int gen_col_width(int n, int nx, size_t cs) {
+  for (int w = 8; w >= 2; w /= 2)
+    if (nx % w == 0 && gen_cols_lds(w, n, cs) <= 64 * 1024) return w;
+  return 1;
+}
+
+// A compiled shape of pass `kind` for length n whose tile divides `other`:
+// the given default if it does, else the entry with the same radix and the
+// widest tile that does, else any radix. False: none.
+static const KEntry* find_entry(int kind, int prec, int n, int e, int p);
+static bool fit_shape(int kind, int prec, int n, int other, int* e, int* p) {
+  if (!pow2_in_range(n)) return false;
+  if (find_entry(kind, prec, n, *e, *p) && other % *p == 0) return true;
+  const KTable t = prec == SGPE_C128 ? registry_f64(kind) : registry_f32(kind);
+  const KEntry* best = nullptr;
+  for (int pass = 0; pass < 2 && !best; ++pass)
+    for (int i = 0; i < t.count; ++i) {
+      const KEntry& k = t.entries[i];
+      if (k.n != n || other % k.p || (pass == 0 && k.e != *e)) continue;
+      if (!best || k.p > best->p) best = &k;
+    }
+  if (!best) return false;
+  *e = best->e;
+  *p = best->p;
+  return true;
+}
+
 // Bound of one wait of the persistent kernel: 2 s of the 100 MHz wall clock
 // (a pass takes microseconds to a few milliseconds).
 static const long long kDefaultWaitTicks = 200000000LL;
@@ -248,6 +310,56 @@ static int ctx_twiddles(sgpe_ctx* ctx, int prec, int n, int e, const void** out)
   return SGPE_OK;
 }
 
+// Roots of unity root[m] = exp(-2 pi i m / n), m < n, of the runtime-length
+// kernels (any even n): angles reduced exactly to [0, pi/4] (4m mod n and
+// its complement are integers), long double cos/sin there, then the
+// quarter-turn rotation; quarter turns come out exact.
+static int ctx_roots(sgpe_ctx* ctx, int prec, int n, const void** out) {
+  const int key = (prec * 8192 + n) * 32;  // e = 0: no compiled shape uses it
+  auto it = ctx->tw.find(key);
+  if (it != ctx->tw.end()) {
+    *out = it->second;
+    return SGPE_OK;
+  }
+  std::vector<char> host(n * csize(prec));
+  const long double half_pi = 1.5707963267948966192313216916397514L;
+  for (int m = 0; m < n; ++m) {
+    const long long u = 4LL * m;      // angle = (pi / 2) u / n
+    const int quad = (int)(u / n);    // whole quarter turns
+    const long long rem = u - (long long)quad * n;
+    long double c, sn;                // cos, sin of (pi / 2) rem / n
+    if (2 * rem <= n) {
+      const long double a = half_pi * (long double)rem / (long double)n;
+      c = cosl(a);
+      sn = sinl(a);
+    } else {
+      const long double a = half_pi * (long double)(n - rem) / (long double)n;
+      c = sinl(a);
+      sn = cosl(a);
+    }
+    for (int r = 0; r < quad; ++r) {  // rotate by pi / 2: (c, s) -> (-s, c)
+      const long double t = c;
+      c = -sn;
+      sn = t;
+    }
+    const double re = (double)c, im = (double)-sn;  // exp(-i angle)
+    if (prec == SGPE_C128) {
+      ((double*)host.data())[2 * m] = re;
+      ((double*)host.data())[2 * m + 1] = im;
+    } else {
+      ((float*)host.data())[2 * m] = (float)re;
+      ((float*)host.data())[2 * m + 1] = (float)im;
+    }
+  }
+  void* d = nullptr;
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (dev_alloc(&d, host.size()) != hipSuccess) return fail(SGPE_E_NOMEM, "root table allocation failed");
+  HIP_TRY(hipMemcpy(d, host.data(), host.size(), hipMemcpyHostToDevice));
+  ctx->tw[key] = d;
+  *out = d;
+  return SGPE_OK;
+}
+
 // Dynamic LDS of a kernel with data_bytes of exchange buffers for n-point
 // transforms: the twiddle table joins it when it fits (tw_in_lds), and the
 // column pass adds its `tables` (the 2 kinetic y-tables) when those fit
@@ -305,6 +417,14 @@ struct sgpe_plan {
   const KEntry* col = nullptr;
   const void* tw_x = nullptr;  // twiddles for nx (rows)
   const void* tw_y = nullptr;  // twiddles for ny (columns)
+  // Runtime-length kernels (sgpe_generic.h) of an axis without a compiled
+  // pass for its length or tiles: gen_x for the row pass (x transforms, row_e
+  // = 0, row_rows = 1), gen_y for the column pass (col_e = 0, col_w from
+  // gen_col_width); their stage radices and root tables.
+  bool gen_x = false, gen_y = false;
+  GenFft gfx{}, gfy{};
+  const void* root_x = nullptr;
+  const void* root_y = nullptr;
   InstDev* d_inst = nullptr;
   RunDev* d_run = nullptr;
   std::vector<InstDev> h_inst;
@@ -319,6 +439,7 @@ struct sgpe_plan {
   int e_col_e = 8, e_col_w = 4, e_row_e = 8, e_row_rows = 1;
   const void* etw_x = nullptr;  // twiddles of the energy kernels' shapes
   const void* etw_y = nullptr;
+  bool e_gen_x = false, e_gen_y = false;  // energy passes on the runtime-length kernels
   // Persistent step kernel (sgpe_persist.h): preference, the entry for the
   // current shapes (null: none compiled or not resident), its grid and LDS.
   bool persist = false;
@@ -338,37 +459,66 @@ static void drop_graphs(sgpe_plan* p) {
   ++p->epoch;
 }
 
+// Pass shapes of a plan. row_e = 0 (col_e = 0): the row (column) pass runs
+// on the runtime-length kernels, row_rows (col_w) then ignored.
 static int plan_select(sgpe_plan* p, int row_e, int row_rows, int col_e, int col_w) {
-  const KEntry* r = find_entry(K_ROW, p->prec, p->nx, row_e, row_rows);
-  const KEntry* c = find_entry(K_COL, p->prec, p->ny, col_e, col_w);
-  if (!r || !c) return fail(SGPE_E_UNSUPPORTED, "FFT shape not compiled for this grid/precision");
-  if (p->ny % row_rows || p->nx % col_w) return fail(SGPE_E_UNSUPPORTED, "tile does not divide the grid");
+  const bool gx = row_e == 0, gy = col_e == 0;
+  if ((gx && !gen_length_ok(p->nx)) || (gy && !gen_length_ok(p->ny)))
+    return fail(SGPE_E_UNSUPPORTED, "no compiled FFT tile of this length divides the other grid axis");
   const size_t cs = csize(p->prec);
+  if (gx) row_rows = 1;
+  if (gy) col_w = gen_col_width(p->ny, p->nx, cs);
+  const KEntry* r = gx ? nullptr : find_entry(K_ROW, p->prec, p->nx, row_e, row_rows);
+  const KEntry* c = gy ? nullptr : find_entry(K_COL, p->prec, p->ny, col_e, col_w);
+  if ((!gx && !r) || (!gy && !c)) return fail(SGPE_E_UNSUPPORTED, "FFT shape not compiled for this grid/precision");
+  if (p->ny % row_rows || p->nx % col_w) return fail(SGPE_E_UNSUPPORTED, "tile does not divide the grid");
   const size_t row_data = (size_t)row_rows * 2 * p->nx * cs;
   const size_t col_data = (size_t)col_w * p->ny * cs;
   if (row_data > kMaxLds || col_data > kMaxLds) return fail(SGPE_E_UNSUPPORTED, "FFT shape exceeds LDS");
-  const size_t rl = row_lds_bytes(row_e, row_data, p->nx, cs), cl = col_lds_bytes(col_w, p->ny, cs);
-  int rc = set_lds_attr(r->kernel, rl);
-  if (!rc) rc = set_lds_attr(r->kernel_g, rl);
-  if (!rc) rc = set_lds_attr(c->kernel, cl);
-  if (!rc) rc = set_lds_attr(c->kernel_g, cl);
-  if (rc) return rc;
+  const size_t rl = gx ? gen_rows_lds(2, p->nx, cs) : row_lds_bytes(row_e, row_data, p->nx, cs);
+  const size_t cl = gy ? gen_cols_lds(col_w, p->ny, cs) : col_lds_bytes(col_w, p->ny, cs);
+  if (rl > kMaxLds || cl > kMaxLds) return fail(SGPE_E_UNSUPPORTED, "FFT shape exceeds LDS");
+  int rc = SGPE_OK;
   const void *twx = nullptr, *twy = nullptr;
-  if ((rc = ctx_twiddles(p->ctx, p->prec, p->nx, row_e, &twx))) return rc;
-  if ((rc = ctx_twiddles(p->ctx, p->prec, p->ny, col_e, &twy))) return rc;
-  p->tw_x = twx;
-  p->tw_y = twy;
+  if (gx) {
+    rc = set_lds_attr(p->prec == SGPE_C128 ? (const void*)gen_row_pass_kernel<double>
+                                           : (const void*)gen_row_pass_kernel<float>, rl);
+    if (!rc) rc = ctx_roots(p->ctx, p->prec, p->nx, &twx);
+  } else {
+    rc = set_lds_attr(r->kernel, rl);
+    if (!rc) rc = set_lds_attr(r->kernel_g, rl);
+    if (!rc) rc = ctx_twiddles(p->ctx, p->prec, p->nx, row_e, &twx);
+  }
+  if (rc) return rc;
+  if (gy) {
+    rc = set_lds_attr(p->prec == SGPE_C128 ? (const void*)gen_col_pass_kernel<double>
+                                           : (const void*)gen_col_pass_kernel<float>, cl);
+    if (!rc) rc = ctx_roots(p->ctx, p->prec, p->ny, &twy);
+  } else {
+    rc = set_lds_attr(c->kernel, cl);
+    if (!rc) rc = set_lds_attr(c->kernel_g, cl);
+    if (!rc) rc = ctx_twiddles(p->ctx, p->prec, p->ny, col_e, &twy);
+  }
+  if (rc) return rc;
+  p->gen_x = gx;
+  p->gen_y = gy;
+  p->gfx = gen_factor(p->nx);
+  p->gfy = gen_factor(p->ny);
+  p->tw_x = gx ? nullptr : twx;
+  p->tw_y = gy ? nullptr : twy;
+  p->root_x = gx ? twx : nullptr;
+  p->root_y = gy ? twy : nullptr;
   p->row = r;
   p->col = c;
-  p->row_e = row_e;
+  p->row_e = gx ? 0 : row_e;
   p->row_rows = row_rows;
-  p->col_e = col_e;
+  p->col_e = gy ? 0 : col_e;
   p->col_w = col_w;
   // Persistent kernel of these shapes, if compiled and if every workgroup of
   // its grid can be resident at once (occupancy of the kernel x CUs).
   p->pk = nullptr;
   p->pgrid = 0;
-  const PEntry* pe = p->nx == p->ny && p->nslots <= kPersistMaxSlots
+  const PEntry* pe = !gx && !gy && p->nx == p->ny && p->nslots <= kPersistMaxSlots
                          ? find_persist(p->prec, p->nx, row_e, row_rows, col_e, col_w)
                          : nullptr;
   if (pe) {
@@ -396,8 +546,29 @@ static void* direct_state(const sgpe_plan* p) {
   return p->nslots == 1 && p->slots[0].loaded ? p->h_inst[0].psi : nullptr;
 }
 
+// Launch with start/stop events when given (profiling), else a plain launch
+// (the form graph capture records).
+#define SGPE_LAUNCH_EV(K, g, b, l, s, e0, e1, ...)                        \
+  do {                                                                    \
+    if (e0)                                                               \
+      hipExtLaunchKernelGGL(K, g, b, (uint32_t)(l), s, e0, e1, 0, __VA_ARGS__); \
+    else                                                                  \
+      hipLaunchKernelGGL(K, g, b, l, s, __VA_ARGS__);                     \
+  } while (0)
+
 static void launch_col(sgpe_plan* p, hipStream_t s, int mode, int dprev, int dcur, hipEvent_t e0 = nullptr,
                        hipEvent_t e1 = nullptr) {
+  if (p->gen_y) {
+    const size_t lds = gen_cols_lds(p->col_w, p->ny, csize(p->prec));
+    const dim3 grid(p->nx / p->col_w, 2, p->nslots), block(kGenThreads);
+    if (p->prec == SGPE_C128)
+      SGPE_LAUNCH_EV(gen_col_pass_kernel<double>, grid, block, lds, s, e0, e1, p->d_inst, p->d_run, p->gfy,
+                     (const double2*)p->root_y, p->col_w, mode, dprev, dcur);
+    else
+      SGPE_LAUNCH_EV(gen_col_pass_kernel<float>, grid, block, lds, s, e0, e1, p->d_inst, p->d_run, p->gfy,
+                     (const float2*)p->root_y, p->col_w, mode, dprev, dcur);
+    return;
+  }
   const size_t lds = col_lds_bytes(p->col_w, p->ny, csize(p->prec));
   dim3 grid(p->nx / p->col_w, 2, p->nslots), block(p->col_w * (p->ny / p->col_e));
   ((ColFn)(p->dense_kin ? p->col->launch_g : p->col->launch))(grid, block, lds, s, e0, e1, p->d_inst, p->d_run,
@@ -407,6 +578,17 @@ static void launch_col(sgpe_plan* p, hipStream_t s, int mode, int dprev, int dcu
 
 static void launch_row(sgpe_plan* p, hipStream_t s, int dur, int first, hipEvent_t e0 = nullptr,
                        hipEvent_t e1 = nullptr) {
+  if (p->gen_x) {
+    const size_t lds = gen_rows_lds(2, p->nx, csize(p->prec));
+    const dim3 grid(p->ny, 1, p->nslots), block(kGenThreads);
+    if (p->prec == SGPE_C128)
+      SGPE_LAUNCH_EV(gen_row_pass_kernel<double>, grid, block, lds, s, e0, e1, p->d_inst, p->d_run, p->gfx,
+                     (const double2*)p->root_x, dur, first, p->col_w);
+    else
+      SGPE_LAUNCH_EV(gen_row_pass_kernel<float>, grid, block, lds, s, e0, e1, p->d_inst, p->d_run, p->gfx,
+                     (const float2*)p->root_x, dur, first, p->col_w);
+    return;
+  }
   const size_t lds = row_lds_bytes(p->row_e, (size_t)p->row_rows * 2 * p->nx * csize(p->prec), p->nx, csize(p->prec));
   dim3 grid(p->ny / p->row_rows, 1, p->nslots), block(p->row_rows * (p->nx / p->row_e));
   ((RowFn)(p->general_row ? p->row->launch_g : p->row->launch))(grid, block, lds, s, e0, e1, p->d_inst, p->d_run,
@@ -506,19 +688,66 @@ static int plan_energy_prepare(sgpe_plan* p) {
   if (p->energy_ready) return SGPE_OK;
   const size_t cs = csize(p->prec);
   int rc;
+  const bool f64 = p->prec == SGPE_C128;
+  // Compiled default shapes where one divides the other axis, else the
+  // runtime-length kernels.
   default_col(p->prec, p->ny, &p->e_col_e, &p->e_col_w);
   default_row(p->prec, p->nx, &p->e_row_e, &p->e_row_rows);
-  p->ecol = find_entry(K_ECOL, p->prec, p->ny, p->e_col_e, p->e_col_w);
-  p->erow = find_entry(K_EROW, p->prec, p->nx, p->e_row_e, p->e_row_rows);
-  if (!p->ecol || !p->erow) return fail(SGPE_E_UNSUPPORTED, "energy kernels not compiled for this grid");
-  const size_t cl = lds_bytes((size_t)p->e_col_w * p->ny * cs, p->ny, cs);
-  if ((rc = set_lds_attr(p->ecol->kernel, cl))) return rc;
-  if ((rc = set_lds_attr(p->ecol->kernel_g, cl))) return rc;
-  if ((rc = set_lds_attr(p->erow->kernel, lds_bytes((size_t)p->e_row_rows * 2 * p->nx * cs, p->nx, cs)))) return rc;
-  if ((rc = ctx_twiddles(p->ctx, p->prec, p->ny, p->e_col_e, &p->etw_y))) return rc;
-  if ((rc = ctx_twiddles(p->ctx, p->prec, p->nx, p->e_row_e, &p->etw_x))) return rc;
+  p->e_gen_y = !fit_shape(K_ECOL, p->prec, p->ny, p->nx, &p->e_col_e, &p->e_col_w);
+  p->e_gen_x = !fit_shape(K_EROW, p->prec, p->nx, p->ny, &p->e_row_e, &p->e_row_rows);
+  if ((p->e_gen_y && !gen_length_ok(p->ny)) || (p->e_gen_x && !gen_length_ok(p->nx)))
+    return fail(SGPE_E_UNSUPPORTED, "energy kernels not compiled for this grid");
+  if (p->e_gen_y) {
+    p->e_col_e = 0;
+    p->e_col_w = gen_col_width(p->ny, p->nx, cs);
+    const size_t cl = gen_cols_lds(p->e_col_w, p->ny, cs);
+    if ((rc = set_lds_attr(f64 ? (const void*)gen_energy_col_kernel<double, false>
+                               : (const void*)gen_energy_col_kernel<float, false>, cl)))
+      return rc;
+    if ((rc = set_lds_attr(f64 ? (const void*)gen_energy_col_kernel<double, true>
+                               : (const void*)gen_energy_col_kernel<float, true>, cl)))
+      return rc;
+    if ((rc = ctx_roots(p->ctx, p->prec, p->ny, &p->etw_y))) return rc;
+  } else {
+    p->ecol = find_entry(K_ECOL, p->prec, p->ny, p->e_col_e, p->e_col_w);
+    const size_t cl = lds_bytes((size_t)p->e_col_w * p->ny * cs, p->ny, cs);
+    if ((rc = set_lds_attr(p->ecol->kernel, cl))) return rc;
+    if ((rc = set_lds_attr(p->ecol->kernel_g, cl))) return rc;
+    if ((rc = ctx_twiddles(p->ctx, p->prec, p->ny, p->e_col_e, &p->etw_y))) return rc;
+  }
+  if (p->e_gen_x) {
+    p->e_row_e = 0;
+    p->e_row_rows = 1;
+    if ((rc = set_lds_attr(f64 ? (const void*)gen_energy_row_kernel<double> : (const void*)gen_energy_row_kernel<float>,
+                           gen_rows_lds(2, p->nx, cs))))
+      return rc;
+    if ((rc = ctx_roots(p->ctx, p->prec, p->nx, &p->etw_x))) return rc;
+  } else {
+    p->erow = find_entry(K_EROW, p->prec, p->nx, p->e_row_e, p->e_row_rows);
+    if ((rc = set_lds_attr(p->erow->kernel, lds_bytes((size_t)p->e_row_rows * 2 * p->nx * cs, p->nx, cs)))) return rc;
+    if ((rc = ctx_twiddles(p->ctx, p->prec, p->nx, p->e_row_e, &p->etw_x))) return rc;
+  }
   p->energy_ready = true;
   return SGPE_OK;
+}
+
+// Energy column kernel (FWD: the closing DFT_y that restores psik).
+template <bool FWD> static void launch_energy_col(sgpe_plan* p, hipStream_t s) {
+  const size_t cs = csize(p->prec);
+  const dim3 cg(p->nx / p->e_col_w, 2, p->nslots);
+  if (p->e_gen_y) {
+    const size_t cl = gen_cols_lds(p->e_col_w, p->ny, cs);
+    if (p->prec == SGPE_C128)
+      hipLaunchKernelGGL((gen_energy_col_kernel<double, FWD>), cg, dim3(kGenThreads), cl, s, p->d_inst, p->d_run,
+                         p->gfy, (const double2*)p->etw_y, p->e_col_w);
+    else
+      hipLaunchKernelGGL((gen_energy_col_kernel<float, FWD>), cg, dim3(kGenThreads), cl, s, p->d_inst, p->d_run,
+                         p->gfy, (const float2*)p->etw_y, p->e_col_w);
+    return;
+  }
+  const size_t cl = lds_bytes((size_t)p->e_col_w * p->ny * cs, p->ny, cs);
+  const dim3 cb(p->e_col_w * (p->ny / p->e_col_e));
+  ((EFn)(FWD ? p->ecol->launch_g : p->ecol->launch))(cg, cb, cl, s, p->d_inst, p->d_run, p->etw_y);
 }
 
 // Normalized k-space state, then the energy passes and the per-slot
@@ -526,13 +755,22 @@ static int plan_energy_prepare(sgpe_plan* p) {
 static void issue_energy(sgpe_plan* p, hipStream_t s, double* out) {
   issue_sync(p, s);
   const size_t cs = csize(p->prec);
-  const size_t cl = lds_bytes((size_t)p->e_col_w * p->ny * cs, p->ny, cs);
-  const size_t rl = lds_bytes((size_t)p->e_row_rows * 2 * p->nx * cs, p->nx, cs);
-  dim3 cg(p->nx / p->e_col_w, 2, p->nslots), cb(p->e_col_w * (p->ny / p->e_col_e));
-  dim3 rg(p->ny / p->e_row_rows, 1, p->nslots), rb(p->e_row_rows * (p->nx / p->e_row_e));
-  ((EFn)p->ecol->launch)(cg, cb, cl, s, p->d_inst, p->d_run, p->etw_y);
-  ((EFn)p->erow->launch)(rg, rb, rl, s, p->d_inst, p->d_run, p->etw_x);
-  ((EFn)p->ecol->launch_g)(cg, cb, cl, s, p->d_inst, p->d_run, p->etw_y);  // restores psik
+  launch_energy_col<false>(p, s);
+  const dim3 rg(p->ny / p->e_row_rows, 1, p->nslots);
+  if (p->e_gen_x) {
+    const size_t rl = gen_rows_lds(2, p->nx, cs);
+    if (p->prec == SGPE_C128)
+      hipLaunchKernelGGL(gen_energy_row_kernel<double>, rg, dim3(kGenThreads), rl, s, p->d_inst, p->d_run, p->gfx,
+                         (const double2*)p->etw_x);
+    else
+      hipLaunchKernelGGL(gen_energy_row_kernel<float>, rg, dim3(kGenThreads), rl, s, p->d_inst, p->d_run, p->gfx,
+                         (const float2*)p->etw_x);
+  } else {
+    const size_t rl = lds_bytes((size_t)p->e_row_rows * 2 * p->nx * cs, p->nx, cs);
+    const dim3 rb(p->e_row_rows * (p->nx / p->e_row_e));
+    ((EFn)p->erow->launch)(rg, rb, rl, s, p->d_inst, p->d_run, p->etw_x);
+  }
+  launch_energy_col<true>(p, s);  // restores psik
   hipLaunchKernelGGL(energy_final_kernel, dim3(p->nslots), dim3(256), 0, s, p->d_inst, p->d_run,
                      p->nx / p->e_col_w, p->ny / p->e_row_rows, out);
 }
@@ -862,33 +1100,64 @@ int sgpe_fft2(sgpe_ctx* ctx, int precision, int ny, int nx, int n_arrays, void* 
   return guarded([&]() -> int {
     if (!ctx || !in || !out || n_arrays < 0) return fail(SGPE_E_ARG, "bad arguments");
     if (precision != SGPE_C128 && precision != SGPE_C64) return fail(SGPE_E_ARG, "bad precision");
-    if (!pow2_in_range(nx) || !pow2_in_range(ny))
-      return fail(SGPE_E_UNSUPPORTED, "grid sizes must be powers of two in [64, 4096]");
+    if (!length_ok(nx) || !length_ok(ny)) return fail(SGPE_E_UNSUPPORTED, kLengthMsg);
     HIP_TRY(hipSetDevice(ctx->device));
+    const size_t cs = csize(precision);
+    const bool f64 = precision == SGPE_C128;
     int re, rr, ce, cw;
     default_row(precision, nx, &re, &rr);
     default_col(precision, ny, &ce, &cw);
-    const KEntry* fr = find_entry(K_FROW, precision, nx, re, rr);
-    const KEntry* fc = find_entry(K_FCOL, precision, ny, ce, cw);
-    if (!fr || !fc) return fail(SGPE_E_UNSUPPORTED, "FFT shape not compiled");
-    const size_t rl = lds_bytes((size_t)rr * nx * csize(precision), nx, csize(precision));
-    const size_t cl = lds_bytes((size_t)cw * ny * csize(precision), ny, csize(precision));
-    int rc = set_lds_attr(fr->kernel, rl);
+    // Compiled shapes where one fits, else the runtime-length kernels.
+    const bool gx = !fit_shape(K_FROW, precision, nx, ny, &re, &rr);
+    const bool gy = !fit_shape(K_FCOL, precision, ny, nx, &ce, &cw);
+    if ((gx && !gen_length_ok(nx)) || (gy && !gen_length_ok(ny)))
+      return fail(SGPE_E_UNSUPPORTED, "no compiled FFT tile of this length divides the other grid axis");
+    if (gy) cw = gen_col_width(ny, nx, cs);
+    const KEntry* fr = gx ? nullptr : find_entry(K_FROW, precision, nx, re, rr);
+    const KEntry* fc = gy ? nullptr : find_entry(K_FCOL, precision, ny, ce, cw);
+    const size_t rl = gx ? gen_rows_lds(1, nx, cs) : lds_bytes((size_t)rr * nx * cs, nx, cs);
+    const size_t cl = gy ? gen_cols_lds(cw, ny, cs) : lds_bytes((size_t)cw * ny * cs, ny, cs);
+    const void* grk = f64 ? (const void*)gen_fft_rows_kernel<double> : (const void*)gen_fft_rows_kernel<float>;
+    const void* gck = f64 ? (const void*)gen_fft_cols_kernel<double> : (const void*)gen_fft_cols_kernel<float>;
+    int rc = set_lds_attr(gx ? grk : fr->kernel, rl);
     if (rc) return rc;
-    if ((rc = set_lds_attr(fc->kernel, cl))) return rc;
+    if ((rc = set_lds_attr(gy ? gck : fc->kernel, cl))) return rc;
     const void *twx, *twy;
-    if ((rc = ctx_twiddles(ctx, precision, nx, re, &twx))) return rc;
-    if ((rc = ctx_twiddles(ctx, precision, ny, ce, &twy))) return rc;
+    if ((rc = gx ? ctx_roots(ctx, precision, nx, &twx) : ctx_twiddles(ctx, precision, nx, re, &twx))) return rc;
+    if ((rc = gy ? ctx_roots(ctx, precision, ny, &twy) : ctx_twiddles(ctx, precision, ny, ce, &twy))) return rc;
+    const GenFft gfx = gen_factor(nx), gfy = gen_factor(ny);
     const double c = dx * dy / (2.0 * M_PI);
-    dim3 rg(ny / rr), rb(rr * (nx / re)), cg(nx / cw), cb(cw * (ny / ce));
+    hipStream_t st = ctx->stream;
+    auto rows = [&](const void* src, void* dst, int inv, double scale) {
+      if (!gx) {
+        ((FRowFn)fr->launch)(dim3(ny / rr), dim3(rr * (nx / re)), rl, st, src, dst, inv, 1, scale, twx);
+      } else if (f64) {
+        hipLaunchKernelGGL(gen_fft_rows_kernel<double>, dim3(ny), dim3(kGenThreads), rl, st, (const double2*)src,
+                           (double2*)dst, gfx, (const double2*)twx, inv, scale);
+      } else {
+        hipLaunchKernelGGL(gen_fft_rows_kernel<float>, dim3(ny), dim3(kGenThreads), rl, st, (const float2*)src,
+                           (float2*)dst, gfx, (const float2*)twx, inv, scale);
+      }
+    };
+    auto cols = [&](const void* src, void* dst, int inv) {
+      if (!gy) {
+        ((FColFn)fc->launch)(dim3(nx / cw), dim3(cw * (ny / ce)), cl, st, src, dst, nx, inv, twy);
+      } else if (f64) {
+        hipLaunchKernelGGL(gen_fft_cols_kernel<double>, dim3(nx / cw), dim3(kGenThreads), cl, st,
+                           (const double2*)src, (double2*)dst, gfy, (const double2*)twy, nx, cw, inv);
+      } else {
+        hipLaunchKernelGGL(gen_fft_cols_kernel<float>, dim3(nx / cw), dim3(kGenThreads), cl, st, (const float2*)src,
+                           (float2*)dst, gfy, (const float2*)twy, nx, cw, inv);
+      }
+    };
     for (int i = 0; i < n_arrays; ++i) {
       if (!in[i] || !out[i]) return fail(SGPE_E_ARG, "NULL array");
       if (!inverse) {
-        ((FRowFn)fr->launch)(rg, rb, rl, ctx->stream, in[i], out[i], 0, 1, c, twx);
-        ((FColFn)fc->launch)(cg, cb, cl, ctx->stream, out[i], out[i], nx, 0, twy);
+        rows(in[i], out[i], 0, c);
+        cols(out[i], out[i], 0);
       } else {
-        ((FColFn)fc->launch)(cg, cb, cl, ctx->stream, in[i], out[i], nx, 1, twy);
-        ((FRowFn)fr->launch)(rg, rb, rl, ctx->stream, out[i], out[i], 1, 1, 1.0 / ((double)nx * ny * c), twx);
+        cols(in[i], out[i], 1);
+        rows(out[i], out[i], 1, 1.0 / ((double)nx * ny * c));
       }
     }
     HIP_TRY(hipGetLastError());
@@ -988,8 +1257,7 @@ int sgpe_plan_create(sgpe_ctx* ctx, int precision, int ny, int nx, int n_slots, 
     if (!ctx || !out || n_slots <= 0) return fail(SGPE_E_ARG, "bad arguments");
     *out = nullptr;
     if (precision != SGPE_C128 && precision != SGPE_C64) return fail(SGPE_E_ARG, "bad precision");
-    if (!pow2_in_range(nx) || !pow2_in_range(ny))
-      return fail(SGPE_E_UNSUPPORTED, "grid sizes must be powers of two in [64, 4096]");
+    if (!length_ok(nx) || !length_ok(ny)) return fail(SGPE_E_UNSUPPORTED, kLengthMsg);
     if (n_slots > 65535) return fail(SGPE_E_ARG, "too many slots");
     HIP_TRY(hipSetDevice(ctx->device));
     sgpe_plan* p = new sgpe_plan();
@@ -1001,9 +1269,13 @@ int sgpe_plan_create(sgpe_ctx* ctx, int precision, int ny, int nx, int n_slots, 
     p->row_stream = p->col_stream = n_slots == 1 ? 1 : 0;
     p->h_inst.resize(n_slots);
     p->slots.resize(n_slots);
+    // Compiled default shapes where they fit the grid, else the runtime-length
+    // kernels for that axis (e = 0).
     int re, rr, ce, cw;
     default_row(precision, nx, &re, &rr);
     default_col(precision, ny, &ce, &cw);
+    if (!fit_shape(K_ROW, precision, nx, ny, &re, &rr)) re = 0;
+    if (!fit_shape(K_COL, precision, ny, nx, &ce, &cw)) ce = 0;
     int rc = plan_select(p, re, rr, ce, cw);
     if (!rc && dev_alloc((void**)&p->d_inst, n_slots * sizeof(InstDev)) != hipSuccess)
       rc = fail(SGPE_E_NOMEM, "slot table allocation failed");
