@@ -4,8 +4,10 @@
   against the CPU oracle; state to 1e-10 (relative L2), ground-state energy
   and chemical potential to 1e-12.
 * 512^2 real-time Raman run (lab frame): the first 100 steps against the CPU
-  oracle (1e-11); the whole 20000-step complex128 run against the committed
-  oracle fixture of the same run (state to 1e-10); and the full run through
+  oracle (1e-11); the whole 20000-step complex128 run against two committed
+  oracle fixtures of the same run, which differ only in the oracle's FFT
+  (state within 4 times their distance: rounding alone moves this run by
+  4e-10 over 20000 steps); and the full run through
   size-independent properties: atom number conserved exactly, GP energy
   conserved to the splitting error, complex64 (FP64 reductions) within stated
   fidelity and energy bounds of complex128.
@@ -81,6 +83,9 @@ def test_c1_prefix_matches_oracle(gpu, c1_initial):
 
 
 GOLDEN_C1 = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden', 'c1_oracle_20000.npz')
+# The same oracle run with NumPy's FFT in place of torch's (make_c1_fixture.py
+# --numpy-fft): the two differ only in the rounding of their transforms.
+GOLDEN_C1_NP = os.path.join(os.path.dirname(GOLDEN_C1), 'c1_oracle_20000_npfft.npz')
 
 
 def _fixture_module():
@@ -93,38 +98,60 @@ def _fixture_module():
 
 def test_c1_full_run_matches_oracle_fixture(gpu, tmp_path, monkeypatch):
     """The whole 20000-step 512^2 Raman run (complex128) against the oracle's
-    run of the same inputs (tests/golden/make_c1_fixture.py): at steps 100,
-    1000, 5000, 10000 and 20000 the state on a stride-16 sublattice to 1e-10
-    (relative L2) and its projections on 8 seeded +-1 vectors to 4e-10 of
-    ||psik|| (each projection of an error e has mean square ||e||^2, so this
-    bounds the full state's error), the populations every 10th step to 1e-11
-    of N_at, and the final GP energy and chemical potential to 1e-10."""
+    run of the same inputs, held in two fixtures (tests/golden/
+    make_c1_fixture.py) that differ only in the FFT used by the oracle
+    (torch's, NumPy's). This run amplifies rounding: the two FP64 oracles
+    drift apart from 2e-14 (relative L2) after 100 steps to 1e-11 after
+    10000 and 4e-10 after 20000. At steps 100, 1000, 5000, 10000 and 20000
+    the engine's state must be as close to EACH oracle as the oracles are to
+    each other, within a factor 4 (and 1e-11; the engine's arithmetic
+    differs from the oracles' in more places than theirs from each other:
+    folded shifts, tabulated phases, Parseval norms, fused passes; measured
+    ratios are 0.6 to 2.2 from step 1000 on): on a stride-16 sublattice
+    (relative L2) and in its projections on 8 seeded +-1 vectors (relative
+    to ||psik||; each projection of an error e has mean square ||e||^2, so
+    this bounds the full state's error). Populations every 10th step and
+    the final GP energy and chemical potential likewise (floors 1e-11 of
+    N_at and 1e-10)."""
     monkeypatch.setenv('SPINOR_GPE_DATA', str(tmp_path))
-    print("[configs] 512^2 Raman real time, 20000 steps vs the oracle fixture, c128", flush=True)
-    fx = np.load(GOLDEN_C1)
+    print("[configs] 512^2 Raman real time, 20000 steps vs the oracle fixtures, c128", flush=True)
+    fx, fy = np.load(GOLDEN_C1), np.load(GOLDEN_C1_NP)
+    assert np.array_equal(fx['ck_steps'], fy['ck_steps']) and rel(fx['init_sub'], fy['init_sub']) <= 1e-14
     mk = _fixture_module()
-    ps0, o0 = mk.c0_ground_oracle()  # the oracle's ground state, as the fixture's
+    ps0, o0 = mk.c0_ground_oracle()  # the oracle's ground state, as the fixtures'
     assert rel(np.stack(o0.psik_numpy())[:, ::8, ::8], fx['c0_sub']) <= 1e-12
     ps = wl.c1_spinor(o0.psi_numpy(), ps0.space, name='c1-golden-gpu')
     init = np.stack(ps.psik)
     assert rel(init[:, ::16, ::16], fx['init_sub']) <= 1e-12
     r = mk.rademacher(init.size, int(fx['proj_k']), int(fx['proj_seed']))
+
+    def proj_err(p, f, k):
+        return float(np.max(np.abs(p - f['ck_proj'][k])) / f['ck_norm'][k])
+
     done, pops = 0, []
     for k, ck in enumerate(fx['ck_steps']):
         res, _ = ps.real(1e-3, int(ck) - done, 'cuda', progress=False, block_steps=100)
         pops.append(res.pops['vals'])
         done = int(ck)
         st = np.stack(res.psik)
-        sub = rel(st[:, ::16, ::16], fx['ck_sub'][k])
-        proj = np.max(np.abs(mk.project(st, r) - fx['ck_proj'][k])) / fx['ck_norm'][k]
-        print(f"[configs] c1 step {ck}: sublattice rel-L2 {sub:.3e}, projections {proj:.3e}", flush=True)
-        assert sub <= 1e-10, (ck, sub)
-        assert proj <= 4e-10, (ck, proj)
-    pops = np.concatenate(pops)[int(fx['pops_every']) - 1::int(fx['pops_every'])]
-    assert np.max(np.abs(pops - fx['pops'])) <= 1e-11 * ps.atom_num
-    e_tot, mu = fx['gp'][0], fx['gp'][6]
-    assert res.energy_gp['E_tot'] == pytest.approx(e_tot, rel=1e-10)
-    assert res.energy_gp['mu'] == pytest.approx(mu, rel=1e-10)
+        pj = mk.project(st, r)
+        floor_sub = rel(fy['ck_sub'][k], fx['ck_sub'][k])
+        floor_proj = proj_err(fy['ck_proj'][k], fx, k)
+        sub = [rel(st[:, ::16, ::16], f['ck_sub'][k]) for f in (fx, fy)]
+        proj = [proj_err(pj, f, k) for f in (fx, fy)]
+        print(f"[configs] c1 step {ck}: sublattice rel-L2 {sub[0]:.3e} / {sub[1]:.3e} (oracles "
+              f"{floor_sub:.3e}), projections {proj[0]:.3e} / {proj[1]:.3e} (oracles {floor_proj:.3e})",
+              flush=True)
+        assert max(sub) <= max(4 * floor_sub, 1e-11), (ck, sub, floor_sub)
+        assert max(proj) <= max(4 * floor_proj, 4e-11), (ck, proj, floor_proj)
+    every = int(fx['pops_every'])
+    pops = np.concatenate(pops)[every - 1::every]
+    floor_pops = np.max(np.abs(fx['pops'] - fy['pops']))
+    for f in (fx, fy):
+        assert np.max(np.abs(pops - f['pops'])) <= max(4 * floor_pops, 1e-11 * ps.atom_num)
+        for key, i in (('E_tot', 0), ('mu', 6)):
+            floor_e = abs(fx['gp'][i] - fy['gp'][i]) / abs(fx['gp'][i])
+            assert res.energy_gp[key] == pytest.approx(f['gp'][i], rel=max(4 * floor_e, 1e-10)), key
 
 
 def test_c1_full_run_invariants(gpu, c1_initial):
