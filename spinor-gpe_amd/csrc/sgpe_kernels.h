@@ -25,17 +25,22 @@ namespace sgpe {
 // ---------------------------------------------------------------- helpers
 
 // sin and cos by Cody-Waite reduction x = k pi/2 + r, |r| <= pi/4, with
-// pi/2 split into three parts (the first two with 33 and 30 significant bits,
-// so k * part is exact for |k| < 2^20), then Taylor polynomials in r (terms
+// pi/2 = P1 + P2 + P3 (P1 pi/2 rounded to double, P2 and P3 the successive
+// remainders): each fma(-k, Pi, r) rounds once, so r is within a few 1e-16
+// of x - k pi/2 for |k| < 1e15 (|x| < 1.5e15; beyond, a double no longer
+// resolves a radian of phase and the error grows with |x|). The quadrant is
+// k mod 4, computed exactly in double. Then Taylor polynomials in r (terms
 // through r^17 / r^18 for double, r^9 / r^10 for float: truncation below half
-// an ulp). Within 2 ulp of the correctly rounded values for |x| <= 1e5; larger
-// arguments use the device library.
+// an ulp); within 2 ulp of the correctly rounded values. There is no
+// library fallback: inlined at every call site, the device library's
+// large-argument code made the row pass's pointwise block twice as slow even
+// where it never ran (instruction fetch across it).
 __device__ __forceinline__ double cw_reduce(double x, int* quadrant) {
   const double k = rint(x * 0.6366197723675814);
-  double r = fma(-k, 1.5707963267341256, x);
-  r = fma(-k, 6.077100506303966e-11, r);
-  r = fma(-k, 2.0222662487959506e-21, r);
-  *quadrant = (int)k & 3;
+  double r = fma(-k, 1.5707963267948966, x);
+  r = fma(-k, 6.123233995736766e-17, r);
+  r = fma(-k, -1.4973849048591698e-33, r);
+  *quadrant = (int)fma(-4.0, floor(k * 0.25), k);
   return r;
 }
 
@@ -48,10 +53,6 @@ template <typename R> __device__ __forceinline__ void quadrant_fix(int q, R sr, 
 
 template <typename R> __device__ __forceinline__ void sc(R x, R* s, R* c);
 template <> __device__ __forceinline__ void sc<double>(double x, double* s, double* c) {
-  if (!(fabs(x) <= 1.0e5)) {
-    sincos(x, s, c);
-    return;
-  }
   int q;
   const double r = cw_reduce(x, &q);
   const double r2 = r * r;
@@ -77,10 +78,6 @@ template <> __device__ __forceinline__ void sc<double>(double x, double* s, doub
   quadrant_fix<double>(q, sr, cr, s, c);
 }
 template <> __device__ __forceinline__ void sc<float>(float x, float* s, float* c) {
-  if (!(fabsf(x) <= 1.0e5f)) {
-    sincosf(x, s, c);
-    return;
-  }
   int q;
   float r;
   if (fabsf(x) <= 100.0f) {

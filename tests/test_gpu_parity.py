@@ -168,3 +168,23 @@ def test_convergence_stop(gpu):
     res2, _ = ps.imaginary(dt, 50, 'cuda', progress=False, block_steps=50)
     assert abs(res2.energy_gp['E_tot'] / res.energy_gp['E_tot'] - 1) < 50 * 1e-9 * 2
 
+
+
+@pytest.mark.parametrize('mesh', [(64, 64), (96, 96)])
+def test_large_phase_arguments(gpu, mesh):
+    """Real-time potential phases of up to 2e6 rad (a non-separable potential
+    reaching 1e9 trap units, dt = 2e-3: the device evaluates those phases
+    per point, complex128). The device sin/cos reduces such arguments itself
+    (pi/2 in three parts); the oracle's library sin/cos sees bit-identical
+    arguments (V times the same sub-step duration), so the states agree to
+    rounding. A wrong quadrant or a lost low part of pi/2 would show as an
+    O(1) error; 96^2 runs the runtime-length kernels."""
+    ps, _, _ = CASES['ground'](name=f'bigphase{mesh[0]}', mesh=mesh, r=(8, 8))
+    x, y = ps.space['x_mesh'], ps.space['y_mesh']
+    ps.pot_eng = ps.pot_eng + 1e9 * (0.5 + 0.5 * np.sin(0.7 * x * y + 0.3 * x))
+    dt = 2e-3
+    oracle = OraclePropagator(ps, dt, time='real')
+    ref_pops = oracle.run(5)
+    res, _ = ps.real(dt, 5, 'cuda', progress=False)
+    assert rel(res.psik, oracle.psik_numpy()) < 1e-11
+    assert np.max(np.abs(res.pops['vals'] - ref_pops)) < 1e-11 * ps.atom_num
