@@ -360,8 +360,12 @@ def roofline(res, prec, n):
     achieved = alg / (ms * 1e-3) / 1e9
     row_e, row_rows, col_e, col_w = res['tuning']
     real = 'double' if prec == 'c128' else 'float'
-    kname = (f"row_pass_kernel<{real}, {n}, {row_e}, {row_rows}>" if dom == 'row_pass'
-             else f"col_pass_kernel<{real}, {n}, {col_e}, {col_w}>")
+    if dom == 'col_pass':
+        kname = f"col_pass_kernel<{real}, {n}, {col_e}, {col_w}>"
+    elif row_rows < 0:  # split row layout, -rows rows per workgroup
+        kname = f"row_split_kernel<{real}, {n}, {row_e}, {-row_rows}>"
+    else:
+        kname = f"row_pass_kernel<{real}, {n}, {row_e}, {row_rows}>"
     if res['streaming'][0 if dom == 'row_pass' else 1]:
         kname += ' streaming'
     traffic = measured_traffic(kname)

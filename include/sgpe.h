@@ -177,7 +177,9 @@ int sgpe_plan_set_active(sgpe_plan* plan, int slot, int active);
 
 /* Select FFT shapes: elements per thread and rows per block for the row
  * pass, elements per thread and columns per block for the column pass.
- * 0 keeps the current value. Returns SGPE_E_UNSUPPORTED if not compiled. */
+ * 0 keeps the current value. A negative row_rows selects the split row
+ * layout (two waves per row, one per component) with -row_rows rows per
+ * block. Returns SGPE_E_UNSUPPORTED if not compiled. */
 int sgpe_plan_tune(sgpe_plan* plan, int row_e, int row_rows, int col_e, int col_w);
 
 /* Compiled shapes for one pass (0 = rows: {E, rows per block}, 1 = columns:

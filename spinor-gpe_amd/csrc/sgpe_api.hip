@@ -388,6 +388,15 @@ static size_t row_lds_bytes(int e, size_t data_bytes, int n, size_t cs) {
   return lds_bytes(data_bytes, n, cs) + (row_double_buffer(e, data_bytes, tw) ? data_bytes : 0);
 }
 
+// A row shape's rows per workgroup is negative for the split layout
+// (row_split_kernel: two waves per row, one per component). Its threads per
+// workgroup and dynamic LDS for either layout:
+static int row_block_threads(int e, int rows, int n) { return rows < 0 ? -rows * 2 * (n / e) : rows * (n / e); }
+static size_t row_pass_lds(int e, int rows, int n, size_t cs) {
+  if (rows < 0) return lds_bytes((size_t)(-rows) * 2 * n * cs, n, cs);
+  return row_lds_bytes(e, (size_t)rows * 2 * n * cs, n, cs);
+}
+
 static int ctx_reduction(sgpe_ctx* ctx, size_t count) {
   if (ctx->red_count >= count) return SGPE_OK;
   if (ctx->red) HIP_TRY(hipFree(ctx->red));
@@ -471,11 +480,11 @@ static int plan_select(sgpe_plan* p, int row_e, int row_rows, int col_e, int col
   const KEntry* r = gx ? nullptr : find_entry(K_ROW, p->prec, p->nx, row_e, row_rows);
   const KEntry* c = gy ? nullptr : find_entry(K_COL, p->prec, p->ny, col_e, col_w);
   if ((!gx && !r) || (!gy && !c)) return fail(SGPE_E_UNSUPPORTED, "FFT shape not compiled for this grid/precision");
-  if (p->ny % row_rows || p->nx % col_w) return fail(SGPE_E_UNSUPPORTED, "tile does not divide the grid");
-  const size_t row_data = (size_t)row_rows * 2 * p->nx * cs;
+  if (p->ny % std::abs(row_rows) || p->nx % col_w) return fail(SGPE_E_UNSUPPORTED, "tile does not divide the grid");
+  const size_t row_data = (size_t)std::abs(row_rows) * 2 * p->nx * cs;
   const size_t col_data = (size_t)col_w * p->ny * cs;
   if (row_data > kMaxLds || col_data > kMaxLds) return fail(SGPE_E_UNSUPPORTED, "FFT shape exceeds LDS");
-  const size_t rl = gx ? gen_rows_lds(2, p->nx, cs) : row_lds_bytes(row_e, row_data, p->nx, cs);
+  const size_t rl = gx ? gen_rows_lds(2, p->nx, cs) : row_pass_lds(row_e, row_rows, p->nx, cs);
   const size_t cl = gy ? gen_cols_lds(col_w, p->ny, cs) : col_lds_bytes(col_w, p->ny, cs);
   if (rl > kMaxLds || cl > kMaxLds) return fail(SGPE_E_UNSUPPORTED, "FFT shape exceeds LDS");
   int rc = SGPE_OK;
@@ -518,7 +527,7 @@ static int plan_select(sgpe_plan* p, int row_e, int row_rows, int col_e, int col
   // its grid can be resident at once (occupancy of the kernel x CUs).
   p->pk = nullptr;
   p->pgrid = 0;
-  const PEntry* pe = !gx && !gy && p->nx == p->ny && p->nslots <= kPersistMaxSlots
+  const PEntry* pe = !gx && !gy && row_rows > 0 && p->nx == p->ny && p->nslots <= kPersistMaxSlots
                          ? find_persist(p->prec, p->nx, row_e, row_rows, col_e, col_w)
                          : nullptr;
   if (pe) {
@@ -589,8 +598,8 @@ static void launch_row(sgpe_plan* p, hipStream_t s, int dur, int first, hipEvent
                      (const float2*)p->root_x, dur, first, p->col_w);
     return;
   }
-  const size_t lds = row_lds_bytes(p->row_e, (size_t)p->row_rows * 2 * p->nx * csize(p->prec), p->nx, csize(p->prec));
-  dim3 grid(p->ny / p->row_rows, 1, p->nslots), block(p->row_rows * (p->nx / p->row_e));
+  const size_t lds = row_pass_lds(p->row_e, p->row_rows, p->nx, csize(p->prec));
+  dim3 grid(p->ny / std::abs(p->row_rows), 1, p->nslots), block(row_block_threads(p->row_e, p->row_rows, p->nx));
   ((RowFn)(p->general_row ? p->row->launch_g : p->row->launch))(grid, block, lds, s, e0, e1, p->d_inst, p->d_run,
                                                                 direct_state(p), p->ny, p->tw_x, dur, first, p->col_w,
                                                                 p->row_stream);

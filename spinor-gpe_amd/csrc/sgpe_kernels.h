@@ -777,6 +777,131 @@ __global__ __launch_bounds__(ROWS*(N / E)) void row_pass_kernel(const InstDev* _
   row_item<R, N, E, ROWS, GENERAL, 0>(P, psi, ny, tws, dur, col_w, stream, blockIdx.x, smem, gate);
 }
 
+// Row pass with the two components on separate waves (the split layout):
+// ROWS rows per workgroup, each taken by two waves of T = N/E = 64 threads,
+// wave c transforming component c with exchanges inside the wave (no
+// workgroup barrier). The two waves meet in LDS for the pointwise block,
+// which all 128 threads of the row share (N/128 points each), then each wave
+// transforms its component back. Against the layout above (both components
+// per thread, transforms spread over several waves) this halves the LDS
+// exchanges per transform at 512 points: 10.7 vs 11.8 us per pass for one
+// 512^2 complex128 instance, 9.2 vs 9.5 us in complex64 (same-host A/B). The
+// tuner picks it per grid and batch size; rows per workgroup are encoded as
+// -ROWS in the shape tables.
+template <typename R, int N, int E, int ROWS, bool GENERAL>
+__global__ __launch_bounds__(ROWS * 2 * (N / E)) void row_split_kernel(const InstDev* __restrict__ insts,
+                                                                       RunDev* __restrict__ runs, void* direct_psi,
+                                                                       int direct_n,
+                                                                       const typename Cx<R>::T* __restrict__ tws,
+                                                                       int dur, int first_sub, int col_w, int stream) {
+  typedef typename Cx<R>::T C;
+  typedef SwzView<sizeof(C)> V;
+  constexpr int T = N / E;
+  static_assert(T == 64, "one wave per component");
+  constexpr int TP = 2 * T;   // threads of one row
+  constexpr int EP = N / TP;  // pointwise points per thread
+  extern __shared__ double smem[];
+  const int inst = blockIdx.z;
+  RunDev& run = runs[inst];
+  const InstDev& P = insts[inst];
+  // State address and column length as in row_pass_kernel.
+  const bool direct = direct_psi != nullptr;
+  C* const psi = reinterpret_cast<C*>(direct ? direct_psi : P.psi);
+  const int ny = direct ? direct_n : P.ny;
+  if (!direct && (!run.active || !psi || ny <= 0)) return;
+  const int rw = threadIdx.x / TP;
+  const int y = blockIdx.x * ROWS + rw;
+  const int tid = threadIdx.x % TP;
+  const int comp = tid / T;
+  const int t = tid % T;
+  // LDS: [ROWS][2][N] data (swizzled within each component), then the
+  // twiddles when they fit (tw_in_lds).
+  C* const sm = reinterpret_cast<C*>(smem) + rw * 2 * N;
+  C* const lds = sm + comp * N;
+  C* const row = psi + ((size_t)comp * ny + y) * N;
+  constexpr size_t kData = (size_t)ROWS * 2 * N * sizeof(C);
+  constexpr bool kTw = tw_in_lds(kData, N, sizeof(C));
+  C a[E];
+  if (stream) {
+#pragma unroll
+    for (int j = 0; j < E; ++j) a[j] = ld_stream(row + t + j * T);
+  } else {
+#pragma unroll
+    for (int j = 0; j < E; ++j) a[j] = row[t + j * T];
+  }
+  const double* vd0 = P.v_dense[0];
+  const double* vd1 = P.v_dense[1];
+  C py0 = C{(R)1, (R)0}, py1 = C{(R)1, (R)0};
+  const C* px0 = nullptr;
+  const C* px1 = nullptr;
+  if (!vd0) {
+    py0 = reinterpret_cast<const C*>(P.py[dur][0])[y];
+    px0 = reinterpret_cast<const C*>(P.px[dur][0]);
+  }
+  if (!vd1) {
+    py1 = reinterpret_cast<const C*>(P.py[dur][1])[y];
+    px1 = reinterpret_cast<const C*>(P.px[dur][1]);
+  }
+  const C* eth = reinterpret_cast<const C*>(P.etheta);
+  TableStage<kTw, N, ROWS * TP, C> stw;
+  stw.load(tws);
+  if (direct) {
+    asm volatile("" ::: "memory");
+    if (!run.active) return;
+  }
+  // Block 0 of the first sub-step records the previous full step's
+  // populations and counts the step; the state leaves in hybrid form.
+  const int nt2 = 2 * (N / col_w);
+  if (blockIdx.x == 0) {
+    if (first_sub) {
+      const int step = run.step;
+      record_pops(P, run.pops, run.pops_cap, step, nt2);
+      if (threadIdx.x == 0) run.step = step + 1;
+    }
+    if (threadIdx.x == 0) run.form = FORM_HYBRID;
+  }
+  // Real-space normalization from the column pass's k-space sums (Parseval).
+  const double nrm = wave_sum_strided(P.part + PART_NRM * nt2, nt2);
+  const C* tw = stw.store(reinterpret_cast<C*>(smem) + ROWS * 2 * N);
+  if constexpr (kTw) __syncthreads();  // publishes the twiddles to every wave
+  const double lam2 = P.atom_num / (P.dv_r * (double)N * (double)ny * nrm);
+  fft_run<N, E, true, V, true>(a, t, tw, lds);
+  // The x tables of this thread's pointwise points, loaded here so that they
+  // arrive while the workgroup meets at the barrier (loaded at the start,
+  // they kept the prologue waiting for the slot descriptor's pointers).
+  C tx0[EP], tx1[EP], tex[EP];
+  load_x_tables<EP, TP>(tx0, tx1, tex, px0, px1, eth, tws, tid);
+#pragma unroll
+  for (int j = 0; j < E; ++j) lds[V::at(t + j * T)] = a[j];
+  __syncthreads();
+  C p0[EP], p1[EP];
+#pragma unroll
+  for (int j = 0; j < EP; ++j) {
+    p0[j] = sm[V::at(tid + j * TP)];
+    p1[j] = sm[N + V::at(tid + j * TP)];
+  }
+  if (P.imag)
+    row_pointwise<R, N, EP, TP, true, !GENERAL>(p0, p1, P, dur, y, tid, lam2, py0, py1, tx0, tx1, tex);
+  else
+    row_pointwise<R, N, EP, TP, false, !GENERAL>(p0, p1, P, dur, y, tid, lam2, py0, py1, tx0, tx1, tex);
+#pragma unroll
+  for (int j = 0; j < EP; ++j) {
+    sm[V::at(tid + j * TP)] = p0[j];
+    sm[N + V::at(tid + j * TP)] = p1[j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < E; ++j) a[j] = lds[V::at(t + j * T)];
+  fft_run<N, E, false, V, true>(a, t, tw, lds);
+  if (stream) {
+#pragma unroll
+    for (int j = 0; j < E; ++j) st_stream(row + t + j * T, a[j]);
+  } else {
+#pragma unroll
+    for (int j = 0; j < E; ++j) row[t + j * T] = a[j];
+  }
+}
+
 // --------------------------------------------------------------- normalize
 // k-space renormalization to atom_num (tensor_propagator.py:271) from the
 // exit column pass's S sums, plus the populations of the last step

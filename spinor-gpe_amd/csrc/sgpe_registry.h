@@ -22,7 +22,7 @@ typedef void (*FColFn)(dim3, dim3, size_t, hipStream_t, const void*, void*, int,
 typedef void (*EFn)(dim3, dim3, size_t, hipStream_t, const InstDev*, const RunDev*, const void*);
 
 struct KEntry {
-  int prec, n, e, p;  // p = W (columns per block) or ROWS (rows per block)
+  int prec, n, e, p;  // p = W (columns per block) or ROWS (rows per block; -ROWS: split row layout)
   void* launch;
   const void* kernel;
   void* launch_g;  // split-step passes: the variant for dense operators
@@ -59,6 +59,17 @@ void row_launch(dim3 g, dim3 b, size_t l, hipStream_t s, hipEvent_t e0, hipEvent
   else
     hipLaunchKernelGGL((row_pass_kernel<R, N, E, ROWS, G>), g, b, l, s, i, r, dpsi, dn, (const C*)tw, dur, first, colw,
                        st);
+}
+template <typename R, int N, int E, int ROWS, bool G>
+void rowsplit_launch(dim3 g, dim3 b, size_t l, hipStream_t s, hipEvent_t e0, hipEvent_t e1, const InstDev* i,
+                     RunDev* r, void* dpsi, int dn, const void* tw, int dur, int first, int colw, int st) {
+  typedef typename Cx<R>::T C;
+  if (e0)
+    hipExtLaunchKernelGGL((row_split_kernel<R, N, E, ROWS, G>), g, b, (uint32_t)l, s, e0, e1, 0, i, r, dpsi, dn,
+                          (const C*)tw, dur, first, colw, st);
+  else
+    hipLaunchKernelGGL((row_split_kernel<R, N, E, ROWS, G>), g, b, l, s, i, r, dpsi, dn, (const C*)tw, dur, first,
+                       colw, st);
 }
 template <typename R, int N, int E, int ROWS>
 void frow_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const void* in, void* out, int inv, int shift, double scale,
@@ -101,6 +112,15 @@ void erow_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const InstDev* i, cons
    (const void*)&sgpe::row_pass_kernel<R, N, E, RW, false>,                                          \
    (void*)&sgpe::row_launch<R, N, E, RW, true>,                                                      \
    (const void*)&sgpe::row_pass_kernel<R, N, E, RW, true>},
+#define SGPE_ROWSPLIT_E(R, PR, N, E, RW)                                                             \
+  {PR,                                                                                               \
+   N,                                                                                                \
+   E,                                                                                                \
+   -(RW),                                                                                            \
+   (void*)&sgpe::rowsplit_launch<R, N, E, RW, false>,                                                \
+   (const void*)&sgpe::row_split_kernel<R, N, E, RW, false>,                                         \
+   (void*)&sgpe::rowsplit_launch<R, N, E, RW, true>,                                                 \
+   (const void*)&sgpe::row_split_kernel<R, N, E, RW, true>},
 #define SGPE_FROW_E(R, PR, N, E, RW)                                                                          \
   {PR, N, E, RW, (void*)&sgpe::frow_launch<R, N, E, RW>, (const void*)&sgpe::fft_rows_kernel<R, N, E, RW>, \
    nullptr, nullptr},
@@ -147,6 +167,12 @@ void erow_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const InstDev* i, cons
   X(R, PR, 1024, 16, 1) X(R, PR, 1024, 16, 2) X(R, PR, 2048, 4, 1) X(R, PR, 2048, 8, 1) X(R, PR, 2048, 16, 1) \
   X(R, PR, 4096, 8, 1) X(R, PR, 4096, 16, 1)
 
+// Split row layout (row_split_kernel: one wave of N/E = 64 threads per
+// component) with ROWS rows per workgroup; listed with rows = -ROWS.
+#define SGPE_ROW_SPLIT_SHAPES(X, R, PR)                                                                \
+  X(R, PR, 128, 2, 2) X(R, PR, 128, 2, 4) X(R, PR, 256, 4, 1) X(R, PR, 256, 4, 2) X(R, PR, 512, 8, 1)  \
+  X(R, PR, 512, 8, 2) X(R, PR, 1024, 16, 1) X(R, PR, 1024, 16, 2)
+
 // Default shapes (standalone FFTs and energy evaluation use only these);
 // they must match default_row / default_col in sgpe_api.hip.
 #define SGPE_DEF_ROWS(X, R, PR)                                                                     \
@@ -162,7 +188,8 @@ void erow_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const InstDev* i, cons
 #define SGPE_DEFINE_REGISTRY(FN, R, PR, DEFCOLS, COLONLY)                                            \
   namespace sgpe {                                                                                   \
   static const KEntry k_col[] = {SGPE_COL_SHAPES(SGPE_COL_E, R, PR) COLONLY(SGPE_COL_E, R, PR)};     \
-  static const KEntry k_row[] = {SGPE_ROW_SHAPES(SGPE_ROW_E, R, PR)};                                \
+  static const KEntry k_row[] = {SGPE_ROW_SHAPES(SGPE_ROW_E, R, PR)                                  \
+                                 SGPE_ROW_SPLIT_SHAPES(SGPE_ROWSPLIT_E, R, PR)};                     \
   static const KEntry k_frow[] = {SGPE_DEF_ROWS(SGPE_FROW_E, R, PR)};                                \
   static const KEntry k_fcol[] = {DEFCOLS(SGPE_FCOL_E, R, PR)};                                      \
   static const KEntry k_ecol[] = {DEFCOLS(SGPE_ECOL_E, R, PR)};                                      \

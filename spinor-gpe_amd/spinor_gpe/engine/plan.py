@@ -195,7 +195,9 @@ class Plan:
         self._call('sgpe_plan_tune', int(row_e), int(row_rows), int(col_e), int(col_w))
 
     def tuning(self):
-        """(row E, rows per block, column E, columns per block)."""
+        """(row E, rows per block, column E, columns per block); rows per
+        block < 0: the split row layout with that many rows per block; E = 0:
+        that pass runs on the runtime-length kernels."""
         out = (ctypes.c_int * 4)()
         nat.check(nat.lib().sgpe_plan_get_tuning(self.handle, out))
         return tuple(out)

@@ -23,7 +23,9 @@ def _packaged_table_only(monkeypatch, tmp_path):
 def _instance(kind, prec, n, shape):
     real = 'double' if prec == 'c128' else 'float'
     e, p, stream = (shape[0], shape[1], shape[4]) if kind == 'row' else (shape[2], shape[3], shape[5])
-    return f"{kind}_pass_kernel<{real}, {n}, {e}, {p}>" + (' streaming' if stream else '')
+    name = (f"row_split_kernel<{real}, {n}, {e}, {-p}>" if kind == 'row' and p < 0  # split row layout
+            else f"{kind}_pass_kernel<{real}, {n}, {e}, {p}>")
+    return name + (' streaming' if stream else '')
 
 
 @pytest.mark.parametrize('prec', ['c128', 'c64'])
