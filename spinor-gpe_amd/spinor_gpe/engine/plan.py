@@ -214,8 +214,20 @@ class Plan:
     def set_persistent(self, on, wait_ticks=0):
         """Run blocks of full steps as one persistent launch where available
         (see ``sgpe_plan_set_persistent``); ``wait_ticks`` bounds each wait
-        of a workgroup, in 100 MHz ticks (0: the library default)."""
+        of a workgroup, in 100 MHz ticks (0: the library default). When the
+        plan's current shapes have no persistent kernel (the tuned shapes
+        need not have one), the plan switches to the first compiled shape
+        of its grid that has one and is resident."""
         self._call('sgpe_plan_set_persistent', int(bool(on)), int(wait_ticks))
+        if on and self.persistent()[2] == 0 and self.ny == self.nx:
+            from spinor_gpe.engine import autotune  # pylint: disable=import-outside-toplevel
+            for shape in autotune.persistent_shapes(self.prec, self.nx):
+                try:
+                    self.tune(*shape)
+                except nat.SgpeError:
+                    continue
+                if self.persistent()[2]:
+                    break
 
     def persistent(self):
         """(requested, used for the next block, grid size or 0)."""

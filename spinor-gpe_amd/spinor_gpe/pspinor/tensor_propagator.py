@@ -13,8 +13,8 @@ captured graph), ``progress`` (tqdm bar, default on as in the reference) and
 ``conv_tol`` (> 0: stop once the GP energy changes by at most this relative
 amount per full step, checked every ``block_steps`` steps; the result then
 holds the steps actually run) and ``persistent`` (run each block of steps as
-one launch of the persistent step kernel where one is compiled for the
-grid's shapes).
+one launch of the persistent step kernel, on a shape of the grid that has
+one compiled; slower than the default pass launches on MI355X).
 """
 
 import warnings
