@@ -179,7 +179,9 @@ int sgpe_plan_set_active(sgpe_plan* plan, int slot, int active);
  * pass, elements per thread and columns per block for the column pass.
  * 0 keeps the current value. A negative row_rows selects the split row
  * layout (two waves per row, one per component) with -row_rows rows per
- * block. Returns SGPE_E_UNSUPPORTED if not compiled. */
+ * block; a negative col_w the column pass that reads its y phase tables from
+ * global memory instead of LDS, with -col_w columns per block. Returns
+ * SGPE_E_UNSUPPORTED if not compiled. */
 int sgpe_plan_tune(sgpe_plan* plan, int row_e, int row_rows, int col_e, int col_w);
 
 /* Compiled shapes for one pass (0 = rows: {E, rows per block}, 1 = columns:
@@ -187,8 +189,9 @@ int sgpe_plan_tune(sgpe_plan* plan, int row_e, int row_rows, int col_e, int col_
  * the number available (>= 0) or a negative error. Feeds the autotuner. */
 int sgpe_list_shapes(int pass, int precision, int n, int* out_pairs, int cap);
 
-/* Current shapes: out4 = {row_e, row_rows, col_e, col_w}; row_e = 0 (col_e =
- * 0): that pass runs on the runtime-length kernels. */
+/* Current shapes: out4 = {row_e, row_rows, col_e, col_w}, negative row_rows
+ * and col_w as in sgpe_plan_tune; row_e = 0 (col_e = 0): that pass runs on
+ * the runtime-length kernels. */
 int sgpe_plan_get_tuning(const sgpe_plan* plan, int* out4);
 
 /* Run blocks of full steps (sgpe_full_steps, sgpe_graph_steps, sgpe_group_run)

@@ -376,8 +376,8 @@ static const int kColTables = 2;
 
 // Dynamic LDS of a column pass: exchange buffer, tables, and the separate
 // slot for its end-of-pass sums when it fits (col_sum_separate).
-static size_t col_lds_bytes(int w, int n, size_t cs) {
-  const size_t base = lds_bytes((size_t)w * n * cs, n, cs, kColTables);
+static size_t col_lds_bytes(int w, int n, size_t cs, bool global_tables = false) {
+  const size_t base = lds_bytes((size_t)w * n * cs, n, cs, global_tables ? 0 : kColTables);
   return base + (col_sum_separate(base) ? kColSumBytes : 0);
 }
 
@@ -431,6 +431,9 @@ struct sgpe_plan {
   // = 0, row_rows = 1), gen_y for the column pass (col_e = 0, col_w from
   // gen_col_width); their stage radices and root tables.
   bool gen_x = false, gen_y = false;
+  // Column pass variant with its y phase tables in global memory (listed
+  // with -W columns per block; col_w keeps the positive tile width).
+  bool col_gky = false;
   GenFft gfx{}, gfy{};
   const void* root_x = nullptr;
   const void* root_y = nullptr;
@@ -480,12 +483,14 @@ static int plan_select(sgpe_plan* p, int row_e, int row_rows, int col_e, int col
   const KEntry* r = gx ? nullptr : find_entry(K_ROW, p->prec, p->nx, row_e, row_rows);
   const KEntry* c = gy ? nullptr : find_entry(K_COL, p->prec, p->ny, col_e, col_w);
   if ((!gx && !r) || (!gy && !c)) return fail(SGPE_E_UNSUPPORTED, "FFT shape not compiled for this grid/precision");
+  const bool gky = !gy && col_w < 0;
+  col_w = std::abs(col_w);
   if (p->ny % std::abs(row_rows) || p->nx % col_w) return fail(SGPE_E_UNSUPPORTED, "tile does not divide the grid");
   const size_t row_data = (size_t)std::abs(row_rows) * 2 * p->nx * cs;
   const size_t col_data = (size_t)col_w * p->ny * cs;
   if (row_data > kMaxLds || col_data > kMaxLds) return fail(SGPE_E_UNSUPPORTED, "FFT shape exceeds LDS");
   const size_t rl = gx ? gen_rows_lds(2, p->nx, cs) : row_pass_lds(row_e, row_rows, p->nx, cs);
-  const size_t cl = gy ? gen_cols_lds(col_w, p->ny, cs) : col_lds_bytes(col_w, p->ny, cs);
+  const size_t cl = gy ? gen_cols_lds(col_w, p->ny, cs) : col_lds_bytes(col_w, p->ny, cs, gky);
   if (rl > kMaxLds || cl > kMaxLds) return fail(SGPE_E_UNSUPPORTED, "FFT shape exceeds LDS");
   int rc = SGPE_OK;
   const void *twx = nullptr, *twy = nullptr;
@@ -523,11 +528,12 @@ static int plan_select(sgpe_plan* p, int row_e, int row_rows, int col_e, int col
   p->row_rows = row_rows;
   p->col_e = gy ? 0 : col_e;
   p->col_w = col_w;
+  p->col_gky = gky;
   // Persistent kernel of these shapes, if compiled and if every workgroup of
   // its grid can be resident at once (occupancy of the kernel x CUs).
   p->pk = nullptr;
   p->pgrid = 0;
-  const PEntry* pe = !gx && !gy && row_rows > 0 && p->nx == p->ny && p->nslots <= kPersistMaxSlots
+  const PEntry* pe = !gx && !gy && !gky && row_rows > 0 && p->nx == p->ny && p->nslots <= kPersistMaxSlots
                          ? find_persist(p->prec, p->nx, row_e, row_rows, col_e, col_w)
                          : nullptr;
   if (pe) {
@@ -578,7 +584,7 @@ static void launch_col(sgpe_plan* p, hipStream_t s, int mode, int dprev, int dcu
                      (const float2*)p->root_y, p->col_w, mode, dprev, dcur);
     return;
   }
-  const size_t lds = col_lds_bytes(p->col_w, p->ny, csize(p->prec));
+  const size_t lds = col_lds_bytes(p->col_w, p->ny, csize(p->prec), p->col_gky);
   dim3 grid(p->nx / p->col_w, 2, p->nslots), block(p->col_w * (p->ny / p->col_e));
   ((ColFn)(p->dense_kin ? p->col->launch_g : p->col->launch))(grid, block, lds, s, e0, e1, p->d_inst, p->d_run,
                                                               direct_state(p), p->nx, p->tw_y, mode, dprev, dcur,
@@ -1422,7 +1428,7 @@ int sgpe_plan_tune(sgpe_plan* p, int row_e, int row_rows, int col_e, int col_w) 
     if (!p) return fail(SGPE_E_ARG, "plan is NULL");
     HIP_TRY(hipSetDevice(p->ctx->device));
     return plan_select(p, row_e ? row_e : p->row_e, row_rows ? row_rows : p->row_rows, col_e ? col_e : p->col_e,
-                       col_w ? col_w : p->col_w);
+                       col_w ? col_w : (p->col_gky ? -p->col_w : p->col_w));
   });
 }
 
@@ -1531,7 +1537,7 @@ int sgpe_plan_get_tuning(const sgpe_plan* p, int* out4) {
     out4[0] = p->row_e;
     out4[1] = p->row_rows;
     out4[2] = p->col_e;
-    out4[3] = p->col_w;
+    out4[3] = p->col_gky ? -p->col_w : p->col_w;
     return SGPE_OK;
   });
 }

@@ -270,7 +270,7 @@ __device__ __forceinline__ void col_kinetic(C (&a)[E], int t, int nx, int jx, co
 // else whether the stored state is already in k-space (1) or hybrid (0).
 // DENSE: the plan has some slot with a dense kinetic array (the code then
 // also handles table slots); otherwise every slot uses the separable tables.
-template <typename R, int N, int E, int W, bool DENSE, int CM, class Gate>
+template <typename R, int N, int E, int W, bool DENSE, int CM, class Gate, bool GKY = false>
 __device__ __forceinline__ void col_item(const InstDev& P, typename Cx<R>::T* __restrict__ psi, int nx,
                                          const typename Cx<R>::T* __restrict__ tws, int mode, int d_prev,
                                          int d_cur, int stream, int comp, int tile, double* smem, Gate gate) {
@@ -278,10 +278,12 @@ __device__ __forceinline__ void col_item(const InstDev& P, typename Cx<R>::T* __
   typedef LdsView<W, 1 << 30, 0> V;
   constexpr int T = N / E;
   // LDS: [W*N exchange][N twiddles][N ky_prev][N ky_cur][sums] (tables and
-  // a separate sum slot when they fit)
+  // a separate sum slot when they fit). GKY: the two y phase tables stay in
+  // global memory (cache resident) even where they would fit, which leaves
+  // LDS for more workgroups per CU in batched plans.
   constexpr size_t kData = (size_t)W * N * sizeof(C);
   constexpr bool kTw = tw_in_lds(kData, N, sizeof(C));
-  constexpr bool kKy = kTw && kData + 3 * (size_t)N * sizeof(C) <= kLdsBytes;
+  constexpr bool kKy = !GKY && kTw && kData + 3 * (size_t)N * sizeof(C) <= kLdsBytes;
   constexpr size_t kBase = kData + (kTw ? (size_t)N * sizeof(C) : 0) + (kKy ? 2 * (size_t)N * sizeof(C) : 0);
   constexpr bool kSumSep = col_sum_separate(kBase);
   // The W*T threads of a tile interleave its W columns, so a column's
@@ -383,7 +385,7 @@ __device__ __forceinline__ void col_item(const InstDev& P, typename Cx<R>::T* __
   }
 }
 
-template <typename R, int N, int E, int W, bool DENSE>
+template <typename R, int N, int E, int W, bool DENSE, bool GKY = false>
 __global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __restrict__ insts,
                                                              RunDev* __restrict__ runs, void* direct_psi,
                                                              int direct_n, const typename Cx<R>::T* __restrict__ tws,
@@ -411,7 +413,7 @@ __global__ __launch_bounds__(W*(N / E)) void col_pass_kernel(const InstDev* __re
     }
     return runs[inst].form == FORM_KSPACE ? 1 : 0;
   };
-  col_item<R, N, E, W, DENSE, 0>(P, psi, nx, tws, mode, d_prev, d_cur, stream, blockIdx.y,
+  col_item<R, N, E, W, DENSE, 0, decltype(gate), GKY>(P, psi, nx, tws, mode, d_prev, d_cur, stream, blockIdx.y,
                                      xcd_tile(blockIdx.x, nx / W), smem, gate);
 }
 

@@ -39,15 +39,16 @@ enum { K_COL = 0, K_ROW = 1, K_FROW = 2, K_FCOL = 3, K_ECOL = 4, K_EROW = 5, K_K
 KTable registry_f64(int kind);
 KTable registry_f32(int kind);
 
-template <typename R, int N, int E, int W, bool G>
+template <typename R, int N, int E, int W, bool G, bool GKY = false>
 void col_launch(dim3 g, dim3 b, size_t l, hipStream_t s, hipEvent_t e0, hipEvent_t e1, const InstDev* i, RunDev* r,
                 void* dpsi, int dn, const void* tw, int m, int dp, int dc, int st) {
   typedef typename Cx<R>::T C;
   if (e0)
-    hipExtLaunchKernelGGL((col_pass_kernel<R, N, E, W, G>), g, b, (uint32_t)l, s, e0, e1, 0, i, r, dpsi, dn,
+    hipExtLaunchKernelGGL((col_pass_kernel<R, N, E, W, G, GKY>), g, b, (uint32_t)l, s, e0, e1, 0, i, r, dpsi, dn,
                           (const C*)tw, m, dp, dc, st);
   else
-    hipLaunchKernelGGL((col_pass_kernel<R, N, E, W, G>), g, b, l, s, i, r, dpsi, dn, (const C*)tw, m, dp, dc, st);
+    hipLaunchKernelGGL((col_pass_kernel<R, N, E, W, G, GKY>), g, b, l, s, i, r, dpsi, dn, (const C*)tw, m, dp, dc,
+                       st);
 }
 template <typename R, int N, int E, int ROWS, bool G>
 void row_launch(dim3 g, dim3 b, size_t l, hipStream_t s, hipEvent_t e0, hipEvent_t e1, const InstDev* i, RunDev* r,
@@ -112,6 +113,17 @@ void erow_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const InstDev* i, cons
    (const void*)&sgpe::row_pass_kernel<R, N, E, RW, false>,                                          \
    (void*)&sgpe::row_launch<R, N, E, RW, true>,                                                      \
    (const void*)&sgpe::row_pass_kernel<R, N, E, RW, true>},
+// Column variant with the y phase tables in global memory, listed with
+// columns per block -W.
+#define SGPE_COLG_E(R, PR, N, E, W)                                                                  \
+  {PR,                                                                                               \
+   N,                                                                                                \
+   E,                                                                                                \
+   -(W),                                                                                             \
+   (void*)&sgpe::col_launch<R, N, E, W, false, true>,                                                \
+   (const void*)&sgpe::col_pass_kernel<R, N, E, W, false, true>,                                     \
+   (void*)&sgpe::col_launch<R, N, E, W, true, true>,                                                 \
+   (const void*)&sgpe::col_pass_kernel<R, N, E, W, true, true>},
 #define SGPE_ROWSPLIT_E(R, PR, N, E, RW)                                                             \
   {PR,                                                                                               \
    N,                                                                                                \
@@ -167,6 +179,14 @@ void erow_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const InstDev* i, cons
   X(R, PR, 1024, 16, 1) X(R, PR, 1024, 16, 2) X(R, PR, 2048, 4, 1) X(R, PR, 2048, 8, 1) X(R, PR, 2048, 16, 1) \
   X(R, PR, 4096, 8, 1) X(R, PR, 4096, 16, 1)
 
+// Column shapes that also have the global-table variant (listed with -W):
+// the batched shapes where staging the two y tables in LDS can cost a
+// workgroup per CU.
+#define SGPE_COLG_SHAPES(X, R, PR)                                                                     \
+  X(R, PR, 128, 4, 8) X(R, PR, 128, 8, 8) X(R, PR, 128, 8, 16) X(R, PR, 256, 4, 8) X(R, PR, 256, 8, 8)   \
+  X(R, PR, 256, 8, 16) X(R, PR, 512, 8, 4) X(R, PR, 512, 8, 8) X(R, PR, 512, 16, 8) X(R, PR, 1024, 8, 2) \
+  X(R, PR, 1024, 8, 4) X(R, PR, 1024, 16, 4)
+
 // Split row layout (row_split_kernel: one wave of N/E = 64 threads per
 // component) with ROWS rows per workgroup; listed with rows = -ROWS.
 #define SGPE_ROW_SPLIT_SHAPES(X, R, PR)                                                                \
@@ -187,7 +207,8 @@ void erow_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const InstDev* i, cons
 
 #define SGPE_DEFINE_REGISTRY(FN, R, PR, DEFCOLS, COLONLY)                                            \
   namespace sgpe {                                                                                   \
-  static const KEntry k_col[] = {SGPE_COL_SHAPES(SGPE_COL_E, R, PR) COLONLY(SGPE_COL_E, R, PR)};     \
+  static const KEntry k_col[] = {SGPE_COL_SHAPES(SGPE_COL_E, R, PR) COLONLY(SGPE_COL_E, R, PR)      \
+                                 SGPE_COLG_SHAPES(SGPE_COLG_E, R, PR)};                              \
   static const KEntry k_row[] = {SGPE_ROW_SHAPES(SGPE_ROW_E, R, PR)                                  \
                                  SGPE_ROW_SPLIT_SHAPES(SGPE_ROWSPLIT_E, R, PR)};                     \
   static const KEntry k_frow[] = {SGPE_DEF_ROWS(SGPE_FROW_E, R, PR)};                                \
