@@ -15,7 +15,8 @@ seeded inputs, to the tolerances of the power-of-two parity tests:
   the end-of-run energy from the device evaluation;
 * plans mixing a compiled pass on one axis with a runtime-length pass on the
   other, a power-of-two axis forced onto the runtime-length kernels, a
-  batched two-slot plan and the shape report.
+  batched two-slot plan, a batched sweep (concurrent plans in one captured
+  graph with on-device energy checks) and the shape report.
 """
 
 import math
@@ -155,3 +156,30 @@ def test_batched_plan(gpu, prec):
     for st, o in zip(states, (o0, o1)):
         err = rel(st.cpu().numpy(), np.stack(o.psik_numpy()))
         assert err <= tol, err
+
+
+@pytest.mark.parametrize('precision', ['c128', 'c64'])
+def test_sweep_on_runtime_length_grids(gpu, precision):
+    """A batched sweep over grids on the runtime-length kernels (96^2, 160x96)
+    and a compiled one (64^2), one captured graph with concurrent plans and
+    on-device energy checks: 60 steps of every instance against the oracle
+    (c128 1e-11, c64 2e-4), and the device GP energy against the oracle's
+    functional on the engine's own state."""
+    from spinor_gpe.engine.sweep import Sweep
+    spins = [cases.raman_rot(name=f'gs{i}', mesh=mesh, r=r, seed=20 + i)[0]
+             for i, (mesh, r) in enumerate([((96, 96), (16, 16)), ((160, 96), (20, 12)),
+                                            ((64, 64), (16, 16)), ((96, 96), (12, 12))])]
+    sw = Sweep(gpu, precision, conv_tol=0.0, check_every=20, max_steps=60)
+    for ps in spins:
+        sw.add_spin(ps, 0.02)
+    res, _ = sw.run()
+    tol = 1e-11 if precision == 'c128' else 2e-4
+    for ps, r in zip(spins, res):
+        assert r.steps == 60
+        oracle = OraclePropagator(ps, 0.02, time='imag')
+        oracle.run(60)
+        err = rel(r.psik, oracle.psik_numpy())
+        assert err < tol, (r.grid, err)
+        cpl = ps.coupling if ps.is_coupling else None
+        e_own = oe.gp_energy(r.psik, ps.space['dr'], ps.kin_eng_spin, ps.pot_eng_spin, ps.g_sc, cpl, None)
+        assert r.energy['E_tot'] == pytest.approx(e_own['E_tot'], rel=1e-12 if precision == 'c128' else 1e-5)
