@@ -175,6 +175,16 @@ int sgpe_plan_status(sgpe_plan* plan, int* out3, double* energy8);
  * kernel (their blocks exit at once). */
 int sgpe_plan_set_active(sgpe_plan* plan, int slot, int active);
 
+/* Reorder the slots (new slot i is old slot order[i], with its state binding,
+ * tables and run state: step counts, populations buffer, energies,
+ * convergence test) and launch the kernels for the first n_launch slots only;
+ * every slot from n_launch on must be inactive, and keeps its state and
+ * status. No counterpart in the reference (one instance per propagator): a
+ * batched sweep whose slots converge one by one stops launching workgroups
+ * for them, without rebuilding the plan. Slot indices change: callers map
+ * their instances through `order`. Synchronizes the stream. */
+int sgpe_plan_compact(sgpe_plan* plan, const int* order, int n_launch);
+
 /* Select FFT shapes: elements per thread and rows per block for the row
  * pass, elements per thread and columns per block for the column pass.
  * 0 keeps the current value. A negative row_rows selects the split row

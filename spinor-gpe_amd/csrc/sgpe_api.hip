@@ -417,6 +417,7 @@ struct Slot {
 struct sgpe_plan {
   sgpe_ctx* ctx = nullptr;
   int prec = 0, ny = 0, nx = 0, nslots = 0;
+  int nlaunch = 0;  // slots the kernels are launched for: the first nlaunch (sgpe_plan_compact)
   int row_e = 8, row_rows = 1, col_e = 8, col_w = 4;
   int row_stream = 0, col_stream = 0;  // non-temporal state accesses per pass
   unsigned epoch = 0;                  // bumped whenever captured graphs go stale
@@ -575,7 +576,7 @@ static void launch_col(sgpe_plan* p, hipStream_t s, int mode, int dprev, int dcu
                        hipEvent_t e1 = nullptr) {
   if (p->gen_y) {
     const size_t lds = gen_cols_lds(p->col_w, p->ny, csize(p->prec));
-    const dim3 grid(p->nx / p->col_w, 2, p->nslots), block(kGenThreads);
+    const dim3 grid(p->nx / p->col_w, 2, p->nlaunch), block(kGenThreads);
     if (p->prec == SGPE_C128)
       SGPE_LAUNCH_EV(gen_col_pass_kernel<double>, grid, block, lds, s, e0, e1, p->d_inst, p->d_run, p->gfy,
                      (const double2*)p->root_y, p->col_w, mode, dprev, dcur);
@@ -585,7 +586,7 @@ static void launch_col(sgpe_plan* p, hipStream_t s, int mode, int dprev, int dcu
     return;
   }
   const size_t lds = col_lds_bytes(p->col_w, p->ny, csize(p->prec), p->col_gky);
-  dim3 grid(p->nx / p->col_w, 2, p->nslots), block(p->col_w * (p->ny / p->col_e));
+  dim3 grid(p->nx / p->col_w, 2, p->nlaunch), block(p->col_w * (p->ny / p->col_e));
   ((ColFn)(p->dense_kin ? p->col->launch_g : p->col->launch))(grid, block, lds, s, e0, e1, p->d_inst, p->d_run,
                                                               direct_state(p), p->nx, p->tw_y, mode, dprev, dcur,
                                                               p->col_stream);
@@ -595,7 +596,7 @@ static void launch_row(sgpe_plan* p, hipStream_t s, int dur, int first, hipEvent
                        hipEvent_t e1 = nullptr) {
   if (p->gen_x) {
     const size_t lds = gen_rows_lds(2, p->nx, csize(p->prec));
-    const dim3 grid(p->ny, 1, p->nslots), block(kGenThreads);
+    const dim3 grid(p->ny, 1, p->nlaunch), block(kGenThreads);
     if (p->prec == SGPE_C128)
       SGPE_LAUNCH_EV(gen_row_pass_kernel<double>, grid, block, lds, s, e0, e1, p->d_inst, p->d_run, p->gfx,
                      (const double2*)p->root_x, dur, first, p->col_w);
@@ -605,7 +606,7 @@ static void launch_row(sgpe_plan* p, hipStream_t s, int dur, int first, hipEvent
     return;
   }
   const size_t lds = row_pass_lds(p->row_e, p->row_rows, p->nx, csize(p->prec));
-  dim3 grid(p->ny / std::abs(p->row_rows), 1, p->nslots), block(row_block_threads(p->row_e, p->row_rows, p->nx));
+  dim3 grid(p->ny / std::abs(p->row_rows), 1, p->nlaunch), block(row_block_threads(p->row_e, p->row_rows, p->nx));
   ((RowFn)(p->general_row ? p->row->launch_g : p->row->launch))(grid, block, lds, s, e0, e1, p->d_inst, p->d_run,
                                                                 direct_state(p), p->ny, p->tw_x, dur, first, p->col_w,
                                                                 p->row_stream);
@@ -636,7 +637,7 @@ static void issue_persist(sgpe_plan* p, hipStream_t s, int n, hipEvent_t e0 = nu
   a.insts = p->d_inst;
   a.runs = p->d_run;
   a.direct_psi = direct_state(p);
-  a.nslots = p->nslots;
+  a.nslots = p->nlaunch;
   a.n_steps = n;
   a.d_first_prev = p->last_dur;
   a.col_stream = p->col_stream;
@@ -675,7 +676,7 @@ static void issue_sync(sgpe_plan* p, hipStream_t s) {
   launch_col(p, s, COL_EXIT, p->last_dur, p->last_dur);
   const size_t n = (size_t)2 * p->ny * p->nx;
   const int blocks = (int)std::min<size_t>((n + 255) / 256, 1024);
-  dim3 grid(blocks, 1, p->nslots);
+  dim3 grid(blocks, 1, p->nlaunch);
   if (p->prec == SGPE_C128)
     hipLaunchKernelGGL(normalize_kernel<double>, grid, dim3(256), 0, s, p->d_inst, p->d_run, p->col_w);
   else
@@ -749,7 +750,7 @@ static int plan_energy_prepare(sgpe_plan* p) {
 // Energy column kernel (FWD: the closing DFT_y that restores psik).
 template <bool FWD> static void launch_energy_col(sgpe_plan* p, hipStream_t s) {
   const size_t cs = csize(p->prec);
-  const dim3 cg(p->nx / p->e_col_w, 2, p->nslots);
+  const dim3 cg(p->nx / p->e_col_w, 2, p->nlaunch);
   if (p->e_gen_y) {
     const size_t cl = gen_cols_lds(p->e_col_w, p->ny, cs);
     if (p->prec == SGPE_C128)
@@ -771,7 +772,7 @@ static void issue_energy(sgpe_plan* p, hipStream_t s, double* out) {
   issue_sync(p, s);
   const size_t cs = csize(p->prec);
   launch_energy_col<false>(p, s);
-  const dim3 rg(p->ny / p->e_row_rows, 1, p->nslots);
+  const dim3 rg(p->ny / p->e_row_rows, 1, p->nlaunch);
   if (p->e_gen_x) {
     const size_t rl = gen_rows_lds(2, p->nx, cs);
     if (p->prec == SGPE_C128)
@@ -786,7 +787,7 @@ static void issue_energy(sgpe_plan* p, hipStream_t s, double* out) {
     ((EFn)p->erow->launch)(rg, rb, rl, s, p->d_inst, p->d_run, p->etw_x);
   }
   launch_energy_col<true>(p, s);  // restores psik
-  hipLaunchKernelGGL(energy_final_kernel, dim3(p->nslots), dim3(256), 0, s, p->d_inst, p->d_run,
+  hipLaunchKernelGGL(energy_final_kernel, dim3(p->nlaunch), dim3(256), 0, s, p->d_inst, p->d_run,
                      p->nx / p->e_col_w, p->ny / p->e_row_rows, out);
 }
 
@@ -1281,6 +1282,7 @@ int sgpe_plan_create(sgpe_ctx* ctx, int precision, int ny, int nx, int n_slots, 
     p->ny = ny;
     p->nx = nx;
     p->nslots = n_slots;
+    p->nlaunch = n_slots;
     p->row_stream = p->col_stream = n_slots == 1 ? 1 : 0;
     p->h_inst.resize(n_slots);
     p->slots.resize(n_slots);
@@ -1333,6 +1335,7 @@ int sgpe_plan_set_slot(sgpe_plan* p, int slot, const sgpe_inst_desc* d, void* ps
   return guarded([&]() -> int {
     if (!p || !d) return fail(SGPE_E_ARG, "bad arguments");
     if (slot < 0 || slot >= p->nslots) return fail(SGPE_E_ARG, "slot out of range");
+    if (slot >= p->nlaunch) return fail(SGPE_E_ARG, "slot is beyond the launched slots (sgpe_plan_compact)");
     HIP_TRY(hipSetDevice(p->ctx->device));
     return plan_load_slot(p, slot, d, psik);
   });
@@ -1393,6 +1396,7 @@ int sgpe_plan_set_active(sgpe_plan* p, int slot, int active) {
   return guarded([&]() -> int {
     if (!p || slot < 0 || slot >= p->nslots) return fail(SGPE_E_ARG, "bad arguments");
     if (active && !p->slots[slot].loaded) return fail(SGPE_E_ARG, "slot has not been set");
+    if (active && slot >= p->nlaunch) return fail(SGPE_E_ARG, "slot is beyond the launched slots (sgpe_plan_compact)");
     HIP_TRY(hipSetDevice(p->ctx->device));
     // A disabled slot keeps its state as it is; finish the pending sub-step
     // first so that it is frozen as normalized k-space (on re-enabling, the
@@ -1400,6 +1404,41 @@ int sgpe_plan_set_active(sgpe_plan* p, int slot, int active) {
     if (!active && p->pending) issue_sync(p, p->ctx->stream);
     hipLaunchKernelGGL(set_active_kernel, dim3(1), dim3(64), 0, p->ctx->stream, p->d_run, slot, active ? 1 : 0);
     HIP_TRY(hipGetLastError());
+    return SGPE_OK;
+  });
+}
+
+int sgpe_plan_compact(sgpe_plan* p, const int* order, int n_launch) {
+  return guarded([&]() -> int {
+    if (!p || !order) return fail(SGPE_E_ARG, "bad arguments");
+    if (n_launch < 1 || n_launch > p->nslots) return fail(SGPE_E_ARG, "n_launch out of range");
+    std::vector<char> seen(p->nslots, 0);
+    for (int i = 0; i < p->nslots; ++i) {
+      if (order[i] < 0 || order[i] >= p->nslots || seen[order[i]])
+        return fail(SGPE_E_ARG, "order is not a permutation of the slots");
+      seen[order[i]] = 1;
+    }
+    HIP_TRY(hipSetDevice(p->ctx->device));
+    hipStream_t s = p->ctx->stream;
+    std::vector<RunDev> run(p->nslots), moved(p->nslots);
+    HIP_TRY(hipMemcpyAsync(run.data(), p->d_run, p->nslots * sizeof(RunDev), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int i = n_launch; i < p->nslots; ++i)
+      if (run[order[i]].active) return fail(SGPE_E_ARG, "a slot left out of the launches is still active");
+    std::vector<InstDev> inst(p->nslots);
+    std::vector<Slot> slots(p->nslots);
+    for (int i = 0; i < p->nslots; ++i) {
+      moved[i] = run[order[i]];
+      inst[i] = p->h_inst[order[i]];
+      slots[i] = p->slots[order[i]];
+    }
+    HIP_TRY(hipMemcpyAsync(p->d_run, moved.data(), p->nslots * sizeof(RunDev), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(p->d_inst, inst.data(), p->nslots * sizeof(InstDev), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    p->h_inst.swap(inst);
+    p->slots.swap(slots);
+    p->nlaunch = n_launch;
+    drop_graphs(p);
     return SGPE_OK;
   });
 }

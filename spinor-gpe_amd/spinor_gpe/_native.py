@@ -76,6 +76,7 @@ def _declare(lib):
         'sgpe_plan_set_state': (i, [vp, i, vp]),
         'sgpe_plan_set_run': (i, [vp, i, vp, i, d]),
         'sgpe_plan_set_active': (i, [vp, i, i]),
+        'sgpe_plan_compact': (i, [vp, ip, i]),
         'sgpe_plan_status': (i, [vp, ip, _dp]),
         'sgpe_plan_tune': (i, [vp, i, i, i, i]),
         'sgpe_list_shapes': (i, [i, i, i, ip, i]),

@@ -141,9 +141,16 @@ def synthetic_spec(ny, nx, time_mode='real'):
     return spec, np.stack([psik, psik])
 
 
-def tune(device, precision, ny, nx, n_slots=1, steps=30, reps=3, verbose=False):
-    """Time every compiled shape; returns (best shape, timings dict in us)."""
-    spec, psik = synthetic_spec(ny, nx)
+def tune(device, precision, ny, nx, n_slots=1, steps=30, reps=3, verbose=False, time_mode=None):
+    """Time every compiled shape; returns (best shape, timings dict in us).
+
+    ``time_mode`` of the synthetic instance: by default real time for one
+    slot (the single-instance real-time runs) and imaginary time for batched
+    plans (ground-state sweeps and served solves); the phase arithmetic of
+    the row pass differs (sin/cos vs exp), and with it the fastest shape."""
+    if time_mode is None:
+        time_mode = 'real' if n_slots == 1 else 'imag'
+    spec, psik = synthetic_spec(ny, nx, time_mode)
     plan = Plan(device, precision, ny, nx, n_slots)
     states = torch.empty((n_slots, 2, ny, nx), dtype=plan.dtype, device=plan.device)
     states.copy_(torch.as_tensor(np.broadcast_to(psik, (n_slots,) + psik.shape).copy(),

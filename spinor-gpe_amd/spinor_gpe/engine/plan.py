@@ -133,6 +133,7 @@ class Plan:
         self.handle = handle
         self.states = [None] * self.n_slots
         self.run_pops = [None] * self.n_slots
+        self.n_launch = self.n_slots  # kernels run for the first n_launch slots (compact)
         if not os.environ.get('SGPE_NO_TUNE_CACHE'):
             from spinor_gpe.engine import autotune  # pylint: disable=import-outside-toplevel
             autotune.apply(self)
@@ -190,6 +191,17 @@ class Plan:
 
     def set_active(self, slot, active):
         self._call('sgpe_plan_set_active', slot, int(bool(active)))
+
+    def compact(self, order, n_launch):
+        """New slot i is old slot ``order[i]`` (state, tables and run state
+        move with it); kernels run for the first ``n_launch`` slots, and the
+        slots after them must be inactive (see ``sgpe_plan_compact``)."""
+        order = [int(o) for o in order]
+        arr = (ctypes.c_int * len(order))(*order)
+        self._call('sgpe_plan_compact', arr, int(n_launch))
+        self.states = [self.states[o] for o in order]
+        self.run_pops = [self.run_pops[o] for o in order]
+        self.n_launch = int(n_launch)
 
     def tune(self, row_e=0, row_rows=0, col_e=0, col_w=0):
         self._call('sgpe_plan_tune', int(row_e), int(row_rows), int(col_e), int(col_w))

@@ -7,7 +7,11 @@ are grouped by grid into plans (one batched launch per pass per plan), all
 plans advance together through one captured hipGraph per check interval
 (concurrent branches, k full steps plus an on-device energy evaluation and
 convergence test each), and converged instances freeze on the device while
-the others continue. The host only polls a small status array.
+the others continue. The host only polls a small status array; when a
+plan's active instances fall to half of the slots it launches, the plan is
+compacted (``Plan.compact``: active slots first, kernels launched for those
+only, run state moved with each slot), so finished instances stop costing
+workgroups.
 """
 
 import ctypes
@@ -121,12 +125,27 @@ class Sweep:
             n = min(self.poll_every, -(-(self.max_steps - steps) // self.check_every))
             group.run(self.check_every, True, n)
             steps += n * self.check_every
-            active = {g: int(plans[g].status()[0][:, 0].sum()) for g in live}
+            flags = {g: plans[g].status()[0][:, 0] for g in live}
+            active = {g: int(f.sum()) for g, f in flags.items()}
             self.trace.append((time.perf_counter() - t0, steps, sum(active.values())))
             if not any(active.values()):
                 break
+            rebuild = not all(active.values())
+            # a plan whose active slots fell to half of its launched ones
+            # launches those only from now on
+            for g in live:
+                plan = plans[g]
+                if 0 < active[g] <= plan.n_launch // 2:
+                    order = [s for s in range(plan.n_slots) if flags[g][s]]
+                    order += [s for s in range(plan.n_slots) if not flags[g][s]]
+                    plan.compact(order, active[g])
+                    where = {old: new for new, old in enumerate(order)}
+                    for i, (gi, si) in slots.items():
+                        if gi == g:
+                            slots[i] = (g, where[si])
+                    rebuild = True
             # plans whose instances have all converged leave the graph
-            if not all(active.values()):
+            if rebuild:
                 live = [g for g in live if active[g]]
                 group = Group([plans[g] for g in live])
         torch.cuda.synchronize(self.device)
@@ -140,7 +159,7 @@ class Sweep:
             res = SolveResult(i, (spec.ny, spec.nx), int(st[slot, 2]), bool(st[slot, 1]),
                               dict(zip(ENERGY_KEYS, (float(v) for v in en[slot]))))
             if keep_states:
-                res.psik = states[g][slot].cpu().numpy()
+                res.psik = plans[g].states[slot].cpu().numpy()
             results.append(res)
         del group
         return results, wall

@@ -12,6 +12,8 @@
   allocation fails (NOMEM), and the plan still steps correctly.
 * device='cpu', the reference's default, runs on the GPU and hands back host
   tensors.
+* A compacted plan (running slots first, launched alone) steps on exactly as
+  the uncompacted one.
 """
 
 import numpy as np
@@ -106,3 +108,46 @@ def test_cpu_device_argument_runs_on_the_gpu(gpu):
     assert prop.space['dr'].device.type == 'cpu'
     assert rel(np.stack(res.psik), np.stack(oracle.psik_numpy())) <= 1e-12
     assert np.max(np.abs(res.pops['vals'] - ref_pops)) <= 1e-12 * ps.atom_num
+
+
+def test_compact_continues_bit_identically(gpu):
+    """Four slots, two of them stopped after 4 steps; the plan is compacted
+    to the two running slots (reordered) and steps on. The running slots'
+    states, steps and energies equal those of an uncompacted plan, bit for
+    bit; the stopped ones keep theirs; bad permutations and an active slot
+    left out of the launches are refused."""
+    spins = [raman_rot(name=f'cmp{i}', seed=30 + i) for i in range(4)]
+
+    def make():
+        plan = Plan(gpu, 'c128', 64, 64, 4)
+        st = [plan.new_state(ps.psik) for ps, _, _ in spins]
+        for k, (ps, t, dt) in enumerate(spins):
+            plan.set_slot(k, InstanceSpec.from_spin(ps, t, dt), st[k])
+        plan.graph_steps(4, 1)
+        plan.energy()
+        plan.set_active(0, False)
+        plan.set_active(2, False)
+        return plan, st
+
+    ref, ref_st = make()
+    ref.graph_steps(4, 2)
+    e_ref = ref.energy().cpu().numpy()
+    ref.sync()
+    plan, st = make()
+    with pytest.raises(nat.SgpeError, match='permutation'):
+        plan.compact([3, 1, 1, 0], 2)
+    with pytest.raises(nat.SgpeError, match='still active'):
+        plan.compact([3, 0, 1, 2], 2)
+    plan.compact([3, 1, 0, 2], 2)
+    with pytest.raises(nat.SgpeError, match='beyond the launched'):
+        plan.set_active(2, True)
+    plan.graph_steps(4, 2)
+    e = plan.energy().cpu().numpy()
+    plan.sync()
+    status, _ = plan.status()
+    ref_status, _ = ref.status()
+    for new, old in enumerate([3, 1, 0, 2]):
+        np.testing.assert_array_equal(st[old].cpu().numpy(), ref_st[old].cpu().numpy())
+        assert tuple(status[new]) == tuple(ref_status[old])
+        if new < 2:
+            np.testing.assert_array_equal(e[new], e_ref[old])
