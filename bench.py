@@ -20,7 +20,8 @@ reference's path) on a bounded sample of the same workload (and of the 256^2
 ground-state workload). Further legs, reported beside the metric: ``large``
 (the 2048^2 vortex workload, where the passes are bandwidth-bound),
 ``sweep`` (96 batched ground states) and ``serve`` (a continuous-batching
-request stream).
+request stream); on request, ``sizes`` and ``fft`` (the reference's own
+propagation and fft_2d benchmarks on its 13 grids).
 """
 
 import argparse
@@ -472,6 +473,65 @@ def sizes_leg(args, device):
                         'Omega=0, imaginary dt=1/50, complex128', 'grids': out}
 
 
+# The reference's fft_2d medians on a Titan V in ms by log2(points)
+# (benchmark_fft.py, bench_data/TitanV_cuda_fft.npz; 4096^2 did not run).
+TITAN_V_FFT_MS = {12: 0.1433, 13: 0.1177, 14: 0.1188, 15: 0.1438, 16: 0.1421, 17: 0.1171, 18: 0.1221,
+                  19: 0.1259, 20: 0.1853, 21: 1.6537, 22: 2.8285, 23: 4.9641}
+
+
+def fft_leg(args, device):
+    """The reference's FFT benchmark (benchmark_fft.py:21-84): the set-up of
+    benchmark_prop on its 13 grids, one imaginary step, then
+    ``ttools.fft_2d(prop.psik, prop.space['dr'])`` timed call by call (median
+    and MAD). The reference's timer brackets the asynchronous launches
+    without a device synchronize; here every call is followed by one, so the
+    figures include the transforms' completion. Also the device time per call
+    from events around a batch of back-to-back calls."""
+    from spinor_gpe.pspinor import pspinor as spin
+    from spinor_gpe.pspinor import tensor_tools as ttools
+    w = 2 * np.pi * 50
+    out = []
+    for grid in SIZES:
+        n_pts = grid[0] * grid[1]
+        log(f"fft: {grid[0]}x{grid[1]}")
+        ps = spin.PSpinor(f'bf{grid[0]}x{grid[1]}', overwrite=True, atom_num=1e2,
+                          omeg={'x': w, 'y': w, 'z': 40 * w}, g_sc={'uu': 1, 'dd': 1, 'ud': 1.04},
+                          pop_frac=(0.5, 0.5), r_sizes=(8, 8), mesh_points=grid)
+        ps.coupling_setup(wavel=790.1e-9, kin_shift=False)
+        _, prop = ps.imaginary(1 / 50, 1, device, is_sampling=False, progress=False)
+        psik, dr = prop.psik, prop.space['dr']
+        for _ in range(3):
+            ttools.fft_2d(psik, dr)
+        calls = []
+        for _ in range(args.fft_calls):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ttools.fft_2d(psik, dr)
+            torch.cuda.synchronize()
+            calls.append(time.perf_counter() - t0)
+        calls = np.array(calls) * 1e3
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.fft_batch):
+            ttools.fft_2d(psik, dr)
+        e1.record()
+        torch.cuda.synchronize()
+        dev_ms = e0.elapsed_time(e1) / args.fft_batch
+        size = int(round(np.log2(n_pts)))
+        med = float(np.median(calls))
+        row = {'grid': list(grid), 'log2_points': size,
+               'fft_2d_call_ms': {'median': round(med, 5), 'mad': round(float(np.median(np.abs(calls - med))), 6)},
+               'device_ms_per_call': round(dev_ms, 5)}
+        if size in TITAN_V_FFT_MS:
+            row['titan_v_ms'] = TITAN_V_FFT_MS[size]
+            row['speedup_vs_titan_v'] = round(TITAN_V_FFT_MS[size] / med, 2)
+        out.append(row)
+        del prop, psik
+        torch.cuda.empty_cache()
+    return {'workload': 'reference benchmark_fft.py: benchmark_prop set-up, fft_2d of the two-component '
+                        'k-space state, complex128', 'grids': out}
+
+
 def step_leg(args, device):
     """The metric value: 512^2 split-steps/s in complex128 (and complex64)."""
     log("building the 512^2 initial state (256^2 ground state, 4000 imaginary steps)")
@@ -530,14 +590,17 @@ def main():
                     help='re-measure the FFT shapes of the workload instead of using the shipped table')
     ap.add_argument('--precisions', nargs='+', default=['c128', 'c64'], choices=['c128', 'c64'])
     ap.add_argument('--legs', nargs='+', default=['step', 'large', 'sweep', 'serve'],
-                    choices=['step', 'large', 'sweep', 'serve', 'sizes'],
+                    choices=['step', 'large', 'sweep', 'serve', 'sizes', 'fft'],
                     help='step: 512^2 split-step rate (the metric value); large: the 2048^2 '
                          'workload; sweep: batched ground states; serve: continuous-batching '
-                         'request stream; sizes (not run by default): the reference\'s own '
-                         'benchmark workload on its 13 grids, beside its Titan V times')
+                         'request stream; sizes and fft (not run by default): the reference\'s '
+                         'own propagation and fft_2d benchmarks on its 13 grids, beside its '
+                         'Titan V times')
     ap.add_argument('--large-steps', type=int, default=1000)
     ap.add_argument('--sizes-calls', type=int, default=53, help='timed full_step() calls per grid (sizes leg)')
     ap.add_argument('--sizes-blocks', type=int, default=10, help='timed graph blocks per grid (sizes leg)')
+    ap.add_argument('--fft-calls', type=int, default=50, help='timed fft_2d calls per grid (fft leg)')
+    ap.add_argument('--fft-batch', type=int, default=20, help='back-to-back fft_2d calls per event pair (fft leg)')
     ap.add_argument('--large-warmup', type=int, default=100)
     ap.add_argument('--requests', type=int, default=48, help='requests of the served stream')
     ap.add_argument('--load', type=float, default=0.8, help='arrival rate / measured capacity')
@@ -571,6 +634,8 @@ def main():
         line['large'] = large_leg(args, device)
     if 'sizes' in args.legs:
         line['sizes'] = sizes_leg(args, device)
+    if 'fft' in args.legs:
+        line['fft'] = fft_leg(args, device)
     if 'sweep' in args.legs:
         line['sweep'] = sweep_leg(device, args.sweep_check, args.fp64_stride)
     if 'serve' in args.legs:

@@ -1144,36 +1144,46 @@ int sgpe_fft2(sgpe_ctx* ctx, int precision, int ny, int nx, int n_arrays, void* 
     const GenFft gfx = gen_factor(nx), gfy = gen_factor(ny);
     const double c = dx * dy / (2.0 * M_PI);
     hipStream_t st = ctx->stream;
-    auto rows = [&](const void* src, void* dst, int inv, double scale) {
+    // k arrays (1 or 2) of the pair io per launch, array blockIdx.z.
+    auto rows = [&](const FftPair& io, int k, int inv, double scale) {
       if (!gx) {
-        ((FRowFn)fr->launch)(dim3(ny / rr), dim3(rr * (nx / re)), rl, st, src, dst, inv, 1, scale, twx);
+        ((FRowFn)fr->launch)(dim3(ny / rr, 1, k), dim3(rr * (nx / re)), rl, st, io, inv, 1, scale, twx);
       } else if (f64) {
-        hipLaunchKernelGGL(gen_fft_rows_kernel<double>, dim3(ny), dim3(kGenThreads), rl, st, (const double2*)src,
-                           (double2*)dst, gfx, (const double2*)twx, inv, scale);
+        hipLaunchKernelGGL(gen_fft_rows_kernel<double>, dim3(ny, 1, k), dim3(kGenThreads), rl, st, io, gfx,
+                           (const double2*)twx, inv, scale);
       } else {
-        hipLaunchKernelGGL(gen_fft_rows_kernel<float>, dim3(ny), dim3(kGenThreads), rl, st, (const float2*)src,
-                           (float2*)dst, gfx, (const float2*)twx, inv, scale);
+        hipLaunchKernelGGL(gen_fft_rows_kernel<float>, dim3(ny, 1, k), dim3(kGenThreads), rl, st, io, gfx,
+                           (const float2*)twx, inv, scale);
       }
     };
-    auto cols = [&](const void* src, void* dst, int inv) {
+    auto cols = [&](const FftPair& io, int k, int inv) {
       if (!gy) {
-        ((FColFn)fc->launch)(dim3(nx / cw), dim3(cw * (ny / ce)), cl, st, src, dst, nx, inv, twy);
+        ((FColFn)fc->launch)(dim3(nx / cw, 1, k), dim3(cw * (ny / ce)), cl, st, io, nx, inv, twy);
       } else if (f64) {
-        hipLaunchKernelGGL(gen_fft_cols_kernel<double>, dim3(nx / cw), dim3(kGenThreads), cl, st,
-                           (const double2*)src, (double2*)dst, gfy, (const double2*)twy, nx, cw, inv);
+        hipLaunchKernelGGL(gen_fft_cols_kernel<double>, dim3(nx / cw, 1, k), dim3(kGenThreads), cl, st, io, gfy,
+                           (const double2*)twy, nx, cw, inv);
       } else {
-        hipLaunchKernelGGL(gen_fft_cols_kernel<float>, dim3(nx / cw), dim3(kGenThreads), cl, st, (const float2*)src,
-                           (float2*)dst, gfy, (const float2*)twy, nx, cw, inv);
+        hipLaunchKernelGGL(gen_fft_cols_kernel<float>, dim3(nx / cw, 1, k), dim3(kGenThreads), cl, st, io, gfy,
+                           (const float2*)twy, nx, cw, inv);
       }
     };
-    for (int i = 0; i < n_arrays; ++i) {
+    for (int i = 0; i < n_arrays; ++i)
       if (!in[i] || !out[i]) return fail(SGPE_E_ARG, "NULL array");
+    for (int i = 0; i < n_arrays; i += 2) {
+      const int k = std::min(2, n_arrays - i);
+      FftPair src{}, dst{};
+      for (int j = 0; j < k; ++j) {
+        src.in[j] = in[i + j];
+        src.out[j] = out[i + j];
+        dst.in[j] = out[i + j];
+        dst.out[j] = out[i + j];
+      }
       if (!inverse) {
-        rows(in[i], out[i], 0, c);
-        cols(out[i], out[i], 0);
+        rows(src, k, 0, c);
+        cols(dst, k, 0);
       } else {
-        cols(in[i], out[i], 1);
-        rows(out[i], out[i], 1, 1.0 / ((double)nx * ny * c));
+        cols(src, k, 1);
+        rows(dst, k, 1, 1.0 / ((double)nx * ny * c));
       }
     }
     HIP_TRY(hipGetLastError());

@@ -25,6 +25,13 @@ template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
 template <typename C, typename R> __device__ __forceinline__ C cscale(C a, R s) { return C{a.x * s, a.y * s}; }
 template <typename C> __device__ __forceinline__ C cconj(C a) { return C{a.x, -a.y}; }
 
+// Inputs and outputs of the standalone 2D transforms: up to two arrays per
+// launch (the two spinor components), passed by value as a kernel argument.
+struct FftPair {
+  const void* in[2];
+  void* out[2];
+};
+
 // Sub-step durations of the 3-stage composition (tensor_propagator.py:101-103):
 // index 0 = outer (gamma * dt), 1 = inner ((1 - 2 gamma) * dt).
 enum { DUR_OUT = 0, DUR_IN = 1 };

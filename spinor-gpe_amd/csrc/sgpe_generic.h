@@ -468,15 +468,16 @@ __global__ __launch_bounds__(kGenThreads) void gen_energy_row_kernel(const InstD
 }
 
 // ---------------------------------------------------- standalone fft_2d
-// Rows of one array (row blockIdx.x): optional pre- (forward) or post-
-// (inverse) factor (-1)^(x+y) * scale, as fft_rows_kernel.
+// Rows (row blockIdx.x) of array blockIdx.z of the pair: optional pre-
+// (forward) or post- (inverse) factor (-1)^(x+y) * scale, as fft_rows_kernel.
 template <typename R>
-__global__ __launch_bounds__(kGenThreads) void gen_fft_rows_kernel(const typename Cx<R>::T* __restrict__ in,
-                                                                   typename Cx<R>::T* __restrict__ out, GenFft f,
+__global__ __launch_bounds__(kGenThreads) void gen_fft_rows_kernel(FftPair io, GenFft f,
                                                                    const typename Cx<R>::T* __restrict__ root,
                                                                    int inv, double scale) {
   typedef typename Cx<R>::T C;
   extern __shared__ double smem[];
+  const C* in = reinterpret_cast<const C*>(io.in[blockIdx.z]);
+  C* out = reinterpret_cast<C*>(io.out[blockIdx.z]);
   const int N = f.n;
   const int y = blockIdx.x;
   C* A = reinterpret_cast<C*>(smem);
@@ -494,14 +495,16 @@ __global__ __launch_bounds__(kGenThreads) void gen_fft_rows_kernel(const typenam
   }
 }
 
-// Columns jx0 .. jx0 + W - 1 of one array (in place allowed).
+// Columns jx0 .. jx0 + W - 1 of array blockIdx.z of the pair (in place
+// allowed).
 template <typename R>
-__global__ __launch_bounds__(kGenThreads) void gen_fft_cols_kernel(const typename Cx<R>::T* __restrict__ in,
-                                                                   typename Cx<R>::T* __restrict__ out, GenFft f,
+__global__ __launch_bounds__(kGenThreads) void gen_fft_cols_kernel(FftPair io, GenFft f,
                                                                    const typename Cx<R>::T* __restrict__ root,
                                                                    int nx, int W, int inv) {
   typedef typename Cx<R>::T C;
   extern __shared__ double smem[];
+  const C* in = reinterpret_cast<const C*>(io.in[blockIdx.z]);
+  C* out = reinterpret_cast<C*>(io.out[blockIdx.z]);
   const int N = f.n;
   const int jx0 = blockIdx.x * W;
   const int cnt = N * W;

@@ -1164,11 +1164,11 @@ static __global__ __launch_bounds__(256) void energy_final_kernel(const InstDev*
 // Device forms of tensor_tools.fft_2d / ifft_2d (tensor_tools.py:201-258):
 //   fft_2d : fftshift(FFTN(psi)) * c  = DFT_y DFT_x [(-1)^(x+y) c psi]
 //   ifft_2d: IFFTN(ifftshift(psik))/c = (-1)^(x+y) / (Nx Ny c) IDFT_x IDFT_y psik
+// Both kernels transform up to two arrays per launch (the two spinor
+// components), array blockIdx.z of the pair.
 // Rows: optional pre- (forward) or post- (inverse) factor (-1)^(x+y) * scale.
 template <typename R, int N, int E, int ROWS>
-__global__ __launch_bounds__(ROWS*(N / E)) void fft_rows_kernel(const typename Cx<R>::T* __restrict__ in,
-                                                                typename Cx<R>::T* __restrict__ out, int inv,
-                                                                int shift, double scale,
+__global__ __launch_bounds__(ROWS*(N / E)) void fft_rows_kernel(FftPair io, int inv, int shift, double scale,
                                                                 const typename Cx<R>::T* __restrict__ tws) {
   typedef typename Cx<R>::T C;
   typedef LdsView<1, 1 << 30, 0> V;
@@ -1177,8 +1177,8 @@ __global__ __launch_bounds__(ROWS*(N / E)) void fft_rows_kernel(const typename C
   const int row = threadIdx.x / T;
   const int t = threadIdx.x % T;
   const int y = blockIdx.x * ROWS + row;
-  const C* src = in + (size_t)y * N;
-  C* dst = out + (size_t)y * N;
+  const C* src = reinterpret_cast<const C*>(io.in[blockIdx.z]) + (size_t)y * N;
+  C* dst = reinterpret_cast<C*>(io.out[blockIdx.z]) + (size_t)y * N;
   C* lds = reinterpret_cast<C*>(smem) + (size_t)row * N;
   constexpr size_t kData = (size_t)ROWS * N * sizeof(C);
   const C* tw = stage_twiddles<tw_in_lds(kData, N, sizeof(C))>(tws, N, reinterpret_cast<C*>(smem) + ROWS * N);
@@ -1201,13 +1201,14 @@ __global__ __launch_bounds__(ROWS*(N / E)) void fft_rows_kernel(const typename C
 }
 
 template <typename R, int N, int E, int W>
-__global__ __launch_bounds__(W*(N / E)) void fft_cols_kernel(const typename Cx<R>::T* __restrict__ in,
-                                                             typename Cx<R>::T* __restrict__ out, int nx,
-                                                             int inv, const typename Cx<R>::T* __restrict__ tws) {
+__global__ __launch_bounds__(W*(N / E)) void fft_cols_kernel(FftPair io, int nx, int inv,
+                                                             const typename Cx<R>::T* __restrict__ tws) {
   typedef typename Cx<R>::T C;
   typedef LdsView<W, 1 << 30, 0> V;
   constexpr int T = N / E;
   extern __shared__ double smem[];
+  const C* in = reinterpret_cast<const C*>(io.in[blockIdx.z]);
+  C* out = reinterpret_cast<C*>(io.out[blockIdx.z]);
   const int c = threadIdx.x % W;
   const int t = threadIdx.x / W;
   const int jx = blockIdx.x * W + c;

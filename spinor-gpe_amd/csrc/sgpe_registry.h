@@ -17,8 +17,8 @@ typedef void (*ColFn)(dim3, dim3, size_t, hipStream_t, hipEvent_t, hipEvent_t, c
                       const void*, int, int, int, int);
 typedef void (*RowFn)(dim3, dim3, size_t, hipStream_t, hipEvent_t, hipEvent_t, const InstDev*, RunDev*, void*, int,
                       const void*, int, int, int, int);
-typedef void (*FRowFn)(dim3, dim3, size_t, hipStream_t, const void*, void*, int, int, double, const void*);
-typedef void (*FColFn)(dim3, dim3, size_t, hipStream_t, const void*, void*, int, int, const void*);
+typedef void (*FRowFn)(dim3, dim3, size_t, hipStream_t, FftPair, int, int, double, const void*);
+typedef void (*FColFn)(dim3, dim3, size_t, hipStream_t, FftPair, int, int, const void*);
 typedef void (*EFn)(dim3, dim3, size_t, hipStream_t, const InstDev*, const RunDev*, const void*);
 
 struct KEntry {
@@ -73,16 +73,15 @@ void rowsplit_launch(dim3 g, dim3 b, size_t l, hipStream_t s, hipEvent_t e0, hip
                        colw, st);
 }
 template <typename R, int N, int E, int ROWS>
-void frow_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const void* in, void* out, int inv, int shift, double scale,
+void frow_launch(dim3 g, dim3 b, size_t l, hipStream_t s, FftPair io, int inv, int shift, double scale,
                  const void* tw) {
   typedef typename Cx<R>::T C;
-  hipLaunchKernelGGL((fft_rows_kernel<R, N, E, ROWS>), g, b, l, s, (const C*)in, (C*)out, inv, shift, scale,
-                     (const C*)tw);
+  hipLaunchKernelGGL((fft_rows_kernel<R, N, E, ROWS>), g, b, l, s, io, inv, shift, scale, (const C*)tw);
 }
 template <typename R, int N, int E, int W>
-void fcol_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const void* in, void* out, int nx, int inv, const void* tw) {
+void fcol_launch(dim3 g, dim3 b, size_t l, hipStream_t s, FftPair io, int nx, int inv, const void* tw) {
   typedef typename Cx<R>::T C;
-  hipLaunchKernelGGL((fft_cols_kernel<R, N, E, W>), g, b, l, s, (const C*)in, (C*)out, nx, inv, (const C*)tw);
+  hipLaunchKernelGGL((fft_cols_kernel<R, N, E, W>), g, b, l, s, io, nx, inv, (const C*)tw);
 }
 template <typename R, int N, int E, int W, bool FWD>
 void ecol_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const InstDev* i, const RunDev* r, const void* tw) {

@@ -162,9 +162,15 @@ class Context:
         with torch.cuda.device(index):
             check(lib().sgpe_ctx_create(index, None, ctypes.byref(handle)))
         self.handle = handle
+        self._stream = None  # stream last bound (the context's own until the first bind)
 
     @classmethod
     def get(cls, device):
+        if isinstance(device, torch.device) and device.type == 'cuda' and device.index is not None:
+            ctx = cls._by_device.get(device.index)  # the per-call path: a device tensor's device
+            if ctx is not None:
+                ctx.bind_stream()
+                return ctx
         dev = require_device(device)
         ctx = cls._by_device.get(dev.index)
         if ctx is None:
@@ -174,7 +180,9 @@ class Context:
 
     def bind_stream(self):
         stream = torch.cuda.current_stream(self.index).cuda_stream
-        check(lib().sgpe_ctx_set_stream(self.handle, ctypes.c_void_p(stream)))
+        if stream != self._stream:
+            check(lib().sgpe_ctx_set_stream(self.handle, ctypes.c_void_p(stream)))
+            self._stream = stream
 
     def synchronize(self):
         check(lib().sgpe_ctx_synchronize(self.handle))

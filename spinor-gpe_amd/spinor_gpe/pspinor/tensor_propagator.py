@@ -49,7 +49,7 @@ class _DeviceDict(dict):
     def __getitem__(self, key):
         val = super().__getitem__(key)
         if key not in self._made:
-            val = torch.tensor(val, device=self._dev)
+            val = ttools.remember_host_copy(torch.tensor(val, device=self._dev), val)
             super().__setitem__(key, val)
             self._made.add(key)
         return val

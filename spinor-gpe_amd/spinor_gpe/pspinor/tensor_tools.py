@@ -13,10 +13,35 @@ Lists of two arrays are spinor wavefunctions ``[psi_up, psi_down]`` of shape
 """
 
 import operator
+import weakref
 from functools import reduce
 
 import numpy as np
 import torch
+
+# Host arrays that device tensors were copied from, by id(tensor): a weak
+# reference to the tensor (checked by identity), its version counter at the
+# copy and the values. Small parameters (the grid spacings ``dr``) are then
+# read on the host without a device-to-host copy, which would synchronize the
+# stream, for as long as the tensor has not been modified in place. (Tensors
+# cannot be weak dictionary keys: key comparison would be elementwise.)
+_HOST_COPIES = {}
+
+
+def remember_host_copy(tensor, array):
+    """Record that ``tensor`` holds the values of the host array ``array``."""
+    key = id(tensor)
+    _HOST_COPIES[key] = (weakref.ref(tensor), tensor._version,  # pylint: disable=protected-access
+                         np.array(array, copy=True))
+    weakref.finalize(tensor, _HOST_COPIES.pop, key, None)
+    return tensor
+
+
+def _host_copy(tensor):
+    entry = _HOST_COPIES.get(id(tensor))
+    if entry is not None and entry[0]() is tensor and entry[1] == tensor._version:  # pylint: disable=protected-access
+        return entry[2]
+    return None
 
 
 def _native():
@@ -110,7 +135,8 @@ def ifft_1d(psik, delta_r=(1, 1), axis=0) -> list:
 
 def _dr_pair(delta_r):
     if _is_tensor(delta_r):
-        delta_r = delta_r.detach().cpu().numpy()
+        known = _host_copy(delta_r)
+        delta_r = known if known is not None else delta_r.detach().cpu().numpy()
     return float(delta_r[0]), float(delta_r[1])
 
 
