@@ -233,31 +233,37 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
 // the closing K of the previous sub-step (unless the state is already in
 // k-space) with the S sum after it, then the opening K of this sub-step with
 // the NRM sum after it (mid). DENSE: K from the dense T array, else from the
-// separable tables.
+// separable tables. The two block-uniform choices (kspace, mid) are taken
+// once, around whole point loops: tested per point, they put every table
+// read in a block of its own with its own wait, a chain of 2E dependent LDS
+// round trips (≈1 µs of a 512-point column pass). The sums run over the
+// points in the same order either way. DENSE reads T in each loop (the second
+// read hits the cache) rather than keeping E values live across both.
 template <typename R, int E, int T, bool DENSE, bool IMAG, typename C>
 __device__ __forceinline__ void col_kinetic(C (&a)[E], int t, int nx, int jx, const double* __restrict__ td, R hp,
                                             R hc, C kxp, C kxc, const C* kyp, const C* kyc, bool kspace, bool mid,
                                             double& s_acc, double& n_acc) {
+  if (!kspace) {
 #pragma unroll
-  for (int j = 0; j < E; ++j) {
-    const int jy = t + j * T;
-    R tval = (R)0;
-    if constexpr (DENSE) tval = (R)td[(size_t)jy * nx + jx];
-    if (!kspace) {
+    for (int j = 0; j < E; ++j) {
       C k;
       if constexpr (DENSE)
-        k = phase_op<R, IMAG>(tval * hp);
+        k = phase_op<R, IMAG>((R)td[(size_t)(t + j * T) * nx + jx] * hp);
       else
-        k = cmul(kxp, kyp[jy]);
+        k = cmul(kxp, kyp[t + j * T]);
       a[j] = cmul(a[j], k);
     }
-    s_acc += norm2d<R>(a[j]);
-    if (mid) {
+  }
+#pragma unroll
+  for (int j = 0; j < E; ++j) s_acc += norm2d<R>(a[j]);
+  if (mid) {
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
       C k;
       if constexpr (DENSE)
-        k = phase_op<R, IMAG>(tval * hc);
+        k = phase_op<R, IMAG>((R)td[(size_t)(t + j * T) * nx + jx] * hc);
       else
-        k = cmul(kxc, kyc[jy]);
+        k = cmul(kxc, kyc[t + j * T]);
       a[j] = cmul(a[j], k);
       n_acc += norm2d<R>(a[j]);
     }
