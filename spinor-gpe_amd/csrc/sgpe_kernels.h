@@ -202,11 +202,24 @@ __device__ __forceinline__ void st_coherent(double* p, double v) {
 // Sum of n doubles by one wave: lane l adds entries l, l + 64, ..., then a
 // butterfly across the lanes; every lane gets the same total, and every wave
 // that calls it sums in the same order. COH: device-coherent reads.
+// Entries are read four per lane at a time, so that their loads are in flight
+// together (one load per iteration made every entry a dependent memory round
+// trip), and added in the same order as one at a time.
 template <bool COH = false>
 __device__ __forceinline__ double wave_sum_strided(const double* __restrict__ p, int n) {
   const int lane = threadIdx.x & 63;
   double v = 0.0;
-  for (int i = lane; i < n; i += 64) v += COH ? ld_coherent(p + i) : p[i];
+  for (int i = lane; i < n; i += 4 * 64) {
+    double e[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int ik = i + 64 * k;
+      e[k] = ik < n ? (COH ? ld_coherent(p + ik) : p[ik]) : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (i + 64 * k < n) v += e[k];
+  }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
