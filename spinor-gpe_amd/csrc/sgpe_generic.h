@@ -32,6 +32,28 @@ struct GenFft {
   int radix[kGenMaxStages];
 };
 
+// dst[e] = src(e) for e = threadIdx.x, threadIdx.x + blockDim.x, ... < n,
+// four elements per thread read before any is written. Array pointers
+// taken from the slot descriptor are generic addresses, which the compiler
+// must assume may alias LDS, so a plain copy loop waits for every load
+// before its store: a memory round trip per element.
+template <typename C, class Src>
+__device__ __forceinline__ void gen_copy_in(C* dst, int n, Src src) {
+  for (int e0 = threadIdx.x; e0 < n; e0 += 4 * (int)blockDim.x) {
+    C v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int e = e0 + k * (int)blockDim.x;
+      if (e < n) v[k] = src(e);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int e = e0 + k * (int)blockDim.x;
+      if (e < n) dst[e] = v[k];
+    }
+  }
+}
+
 // Small DFT of radix R (3, 5 or 7) from the root table: y_q = sum_r b_r w^{rq}
 // with w = root[n / R] (conjugated for the inverse).
 template <int R, bool INV, typename C>
@@ -188,7 +210,7 @@ __global__ __launch_bounds__(kGenThreads) void gen_col_pass_kernel(const InstDev
   C* A = reinterpret_cast<C*>(smem);
   C* B = A + cnt;
   C* psi = reinterpret_cast<C*>(P.psi) + (size_t)comp * N * nx + jx0;
-  for (int e = threadIdx.x; e < cnt; e += blockDim.x) A[e] = psi[(size_t)(e / W) * nx + (e % W)];
+  gen_copy_in(A, cnt, [&](int e) { return psi[(size_t)(e / W) * nx + (e % W)]; });
   const bool kspace = runs[inst].form == FORM_KSPACE;
   __syncthreads();
   C* cur = kspace ? A : gen_fft<false>(A, B, f, W, 1, W, root);
@@ -202,30 +224,48 @@ __global__ __launch_bounds__(kGenThreads) void gen_col_pass_kernel(const InstDev
   const R hc = (R)(0.5 * P.dt_sub[d_cur]);
   const bool mid = mode == COL_MID;
   double v[2] = {0.0, 0.0};
-  for (int e = threadIdx.x; e < cnt; e += blockDim.x) {
-    const int jy = e / W, jx = jx0 + e % W;
-    C a = cur[e];
-    R tval = (R)0;
-    if (td) tval = (R)td[(size_t)jy * nx + jx];
-    if (!kspace) {
-      C k;
-      if (td)
-        k = P.imag ? phase_op<R, true>(tval * hp) : phase_op<R, false>(tval * hp);
-      else
-        k = cmul(kxp[jx], kyp[jy]);
-      a = cmul(a, k);
+  // Four elements per thread at a time: their table entries are read before
+  // any of them is rewritten (see gen_copy_in); same order of operations.
+  for (int e0 = threadIdx.x; e0 < cnt; e0 += 4 * (int)blockDim.x) {
+    C kp[4], kc[4];
+    R tv[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int e = e0 + i * (int)blockDim.x;
+      if (e >= cnt) continue;
+      const int jy = e / W, jx = jx0 + e % W;
+      if (td) {
+        tv[i] = (R)td[(size_t)jy * nx + jx];
+      } else {
+        kp[i] = cmul(kxp[jx], kyp[jy]);
+        kc[i] = cmul(kxc[jx], kyc[jy]);
+      }
     }
-    v[0] += norm2d<R>(a);
-    if (mid) {
-      C k;
-      if (td)
-        k = P.imag ? phase_op<R, true>(tval * hc) : phase_op<R, false>(tval * hc);
-      else
-        k = cmul(kxc[jx], kyc[jy]);
-      a = cmul(a, k);
-      v[1] += norm2d<R>(a);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int e = e0 + i * (int)blockDim.x;
+      if (e >= cnt) continue;
+      C a = cur[e];
+      if (!kspace) {
+        C k;
+        if (td)
+          k = P.imag ? phase_op<R, true>(tv[i] * hp) : phase_op<R, false>(tv[i] * hp);
+        else
+          k = kp[i];
+        a = cmul(a, k);
+      }
+      v[0] += norm2d<R>(a);
+      if (mid) {
+        C k;
+        if (td)
+          k = P.imag ? phase_op<R, true>(tv[i] * hc) : phase_op<R, false>(tv[i] * hc);
+        else
+          k = kc[i];
+        a = cmul(a, k);
+        v[1] += norm2d<R>(a);
+      }
+      cur[e] = a;
     }
-    cur[e] = a;
   }
   __syncthreads();
   if (mid) cur = gen_fft<true>(cur, oth, f, W, 1, W, root);
@@ -324,10 +364,7 @@ __global__ __launch_bounds__(kGenThreads) void gen_row_pass_kernel(const InstDev
   C* B = A + 2 * N;
   C* r0 = reinterpret_cast<C*>(P.psi) + (size_t)y * N;
   C* r1 = r0 + (size_t)ny * N;
-  for (int x = threadIdx.x; x < N; x += blockDim.x) {
-    A[x] = r0[x];
-    A[N + x] = r1[x];
-  }
+  gen_copy_in(A, 2 * N, [&](int e) { return e < N ? r0[e] : r1[e - N]; });
   const int nt2 = 2 * (N / col_w);
   if (blockIdx.x == 0) {
     if (first_sub) {
@@ -384,15 +421,31 @@ __global__ __launch_bounds__(kGenThreads) void gen_energy_col_kernel(const InstD
   C* psi = reinterpret_cast<C*>(P.psi) + (size_t)comp * N * nx + jx0;
   double v[2] = {0.0, 0.0};
   const double* td = P.t_dense[comp];
-  for (int e = threadIdx.x; e < cnt; e += blockDim.x) {
-    const C a = psi[(size_t)(e / W) * nx + (e % W)];
-    A[e] = a;
-    if constexpr (!FWD) {
-      const int jy = e / W, jx = jx0 + e % W;
-      const double tv = td ? td[(size_t)jy * nx + jx] : P.tx[comp][jx] + P.ty[comp][jy];
-      const double m = norm2d<R>(a);
-      v[0] += tv * m;
-      v[1] += m;
+  // Four elements per thread at a time, read before any is stored (see
+  // gen_copy_in); the sums run in the same order.
+  for (int e0 = threadIdx.x; e0 < cnt; e0 += 4 * (int)blockDim.x) {
+    C a[4];
+    double tv[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int e = e0 + i * (int)blockDim.x;
+      if (e >= cnt) continue;
+      a[i] = psi[(size_t)(e / W) * nx + (e % W)];
+      if constexpr (!FWD) {
+        const int jy = e / W, jx = jx0 + e % W;
+        tv[i] = td ? td[(size_t)jy * nx + jx] : P.tx[comp][jx] + P.ty[comp][jy];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int e = e0 + i * (int)blockDim.x;
+      if (e >= cnt) continue;
+      A[e] = a[i];
+      if constexpr (!FWD) {
+        const double m = norm2d<R>(a[i]);
+        v[0] += tv[i] * m;
+        v[1] += m;
+      }
     }
   }
   __syncthreads();
@@ -424,10 +477,7 @@ __global__ __launch_bounds__(kGenThreads) void gen_energy_row_kernel(const InstD
   C* B = A + 2 * N;
   C* r0 = reinterpret_cast<C*>(P.psi) + (size_t)y * N;
   C* r1 = r0 + (size_t)ny * N;
-  for (int x = threadIdx.x; x < N; x += blockDim.x) {
-    A[x] = r0[x];
-    A[N + x] = r1[x];
-  }
+  gen_copy_in(A, 2 * N, [&](int e) { return e < N ? r0[e] : r1[e - N]; });
   __syncthreads();
   C* cur = gen_fft<true>(A, B, f, 2, N, 1, root);
   C* oth = cur == A ? B : A;
@@ -483,10 +533,10 @@ __global__ __launch_bounds__(kGenThreads) void gen_fft_rows_kernel(FftPair io, G
   C* A = reinterpret_cast<C*>(smem);
   C* B = A + N;
   const R sy = (R)((y & 1) ? -scale : scale);
-  for (int x = threadIdx.x; x < N; x += blockDim.x) {
+  gen_copy_in(A, N, [&](int x) {
     const C a = in[(size_t)y * N + x];
-    A[x] = inv ? a : cscale(a, (x & 1) ? -sy : sy);
-  }
+    return inv ? a : cscale(a, (x & 1) ? -sy : sy);
+  });
   __syncthreads();
   C* cur = inv ? gen_fft<true>(A, B, f, 1, N, 1, root) : gen_fft<false>(A, B, f, 1, N, 1, root);
   for (int x = threadIdx.x; x < N; x += blockDim.x) {
@@ -510,7 +560,7 @@ __global__ __launch_bounds__(kGenThreads) void gen_fft_cols_kernel(FftPair io, G
   const int cnt = N * W;
   C* A = reinterpret_cast<C*>(smem);
   C* B = A + cnt;
-  for (int e = threadIdx.x; e < cnt; e += blockDim.x) A[e] = in[(size_t)(e / W) * nx + jx0 + (e % W)];
+  gen_copy_in(A, cnt, [&](int e) { return in[(size_t)(e / W) * nx + jx0 + (e % W)]; });
   __syncthreads();
   C* cur = inv ? gen_fft<true>(A, B, f, W, 1, W, root) : gen_fft<false>(A, B, f, W, 1, W, root);
   for (int e = threadIdx.x; e < cnt; e += blockDim.x) out[(size_t)(e / W) * nx + jx0 + (e % W)] = cur[e];
