@@ -131,22 +131,41 @@ template <bool IMAG, typename C> __device__ __forceinline__ C mul_phase(C p, C u
 // variants: inside one loop the compiler would evaluate the sin/cos or exp
 // of every alternative for every point and select afterwards.
 
-// State accesses of the split-step passes with a non-temporal hint: every
-// pass reads and writes each state element exactly once, and streaming them
-// past the caches shortens a pass when a single instance leaves the chip
-// mostly idle (for batched plans the normal policy can be faster; the plan
-// chooses, see the streaming flags of sgpe_plan_set_streaming).
+// Loads and stores of one complex value through the global address space
+// (global_load / global_store, counted by vmcnt only). The passes' state and
+// table pointers come from slot descriptors (or a select with one), which the
+// compiler would access with flat instructions, and flat accesses count
+// against lgkmcnt as well: every later wait for a scalar load or for LDS
+// traffic then also waited for the state loads to arrive (or for the
+// stores to complete).
+template <typename C> __device__ __forceinline__ C ld_global(const C* p) {
+  typedef decltype(p->x) S;
+  typedef S V2 __attribute__((ext_vector_type(2)));
+  const V2 w = *(const __attribute__((address_space(1))) V2*)p;
+  return C{w.x, w.y};
+}
+template <typename C> __device__ __forceinline__ void st_global(C* p, C v) {
+  typedef decltype(v.x) S;
+  typedef S V2 __attribute__((ext_vector_type(2)));
+  *(__attribute__((address_space(1))) V2*)p = V2{v.x, v.y};
+}
+
+// The same with a non-temporal hint: every pass reads and writes each state
+// element exactly once, and streaming them past the caches shortens a pass
+// when a single instance leaves the chip mostly idle (for batched plans the
+// normal policy can be faster; the plan chooses, see the streaming flags of
+// sgpe_plan_set_streaming).
 template <typename C> __device__ __forceinline__ C ld_stream(const C* p) {
   typedef decltype(p->x) S;
   typedef S V2 __attribute__((ext_vector_type(2)));
-  const V2 w = __builtin_nontemporal_load(reinterpret_cast<const V2*>(p));
+  const V2 w = __builtin_nontemporal_load((const __attribute__((address_space(1))) V2*)p);
   return C{w.x, w.y};
 }
 template <typename C> __device__ __forceinline__ void st_stream(C* p, C v) {
   typedef decltype(v.x) S;
   typedef S V2 __attribute__((ext_vector_type(2)));
   const V2 w = {v.x, v.y};
-  __builtin_nontemporal_store(w, reinterpret_cast<V2*>(p));
+  __builtin_nontemporal_store(w, (__attribute__((address_space(1))) V2*)p);
 }
 
 // Device-coherent accesses (cache policy SC1: device scope) to data that
@@ -325,7 +344,7 @@ __device__ __forceinline__ void col_item(const InstDev& P, typename Cx<R>::T* __
     for (int j = 0; j < E; ++j) a[j] = ld_stream(col + (size_t)(t + j * T) * nx);
   } else {
 #pragma unroll
-    for (int j = 0; j < E; ++j) a[j] = col[(size_t)(t + j * T) * nx];
+    for (int j = 0; j < E; ++j) a[j] = ld_global(col + (size_t)(t + j * T) * nx);
   }
   const double* td = P.t_dense[comp];
   C kxp = C{(R)1, (R)0}, kxc = C{(R)1, (R)0};
@@ -376,7 +395,7 @@ __device__ __forceinline__ void col_item(const InstDev& P, typename Cx<R>::T* __
     for (int j = 0; j < E; ++j) st_stream(col + (size_t)(t + j * T) * nx, a[j]);
   } else {
 #pragma unroll
-    for (int j = 0; j < E; ++j) col[(size_t)(t + j * T) * nx] = a[j];
+    for (int j = 0; j < E; ++j) st_global(col + (size_t)(t + j * T) * nx, a[j]);
   }
 
   double v[2] = {s_acc, n_acc};
@@ -539,7 +558,8 @@ __device__ __forceinline__ void row_pointwise(C (&a0)[E], C (&a1)[E], const Inst
 
 // The x-table values (potential phases of both components, e^{-i theta})
 // of this thread's points J0 <= j < J1 of a row; `stand_in` replaces an
-// absent potential table.
+// absent potential table. Global loads: issued before a transform's last
+// LDS exchange, they stay in flight across its waits.
 template <int E, int T, int J0 = 0, int J1 = E, typename C>
 __device__ __forceinline__ void load_x_tables(C (&tx0)[E], C (&tx1)[E], C (&tex)[E], const C* px0, const C* px1,
                                               const C* eth, const C* stand_in, int t) {
@@ -547,9 +567,9 @@ __device__ __forceinline__ void load_x_tables(C (&tx0)[E], C (&tx1)[E], C (&tex)
   const C* s1 = px1 ? px1 : stand_in;
 #pragma unroll
   for (int j = J0; j < J1; ++j) {
-    tx0[j] = s0[t + j * T];
-    tx1[j] = s1[t + j * T];
-    tex[j] = eth[t + j * T];
+    tx0[j] = ld_global(s0 + t + j * T);
+    tx1[j] = ld_global(s1 + t + j * T);
+    tex[j] = ld_global(eth + t + j * T);
   }
 }
 
@@ -642,8 +662,8 @@ __device__ __forceinline__ void row_item(const InstDev& P, typename Cx<R>::T* __
   } else {
 #pragma unroll
     for (int j = 0; j < E; ++j) {
-      a0[j] = r0[t + j * T];
-      a1[j] = r1[t + j * T];
+      a0[j] = ld_global(r0 + t + j * T);
+      a1[j] = ld_global(r1 + t + j * T);
     }
   }
   constexpr int kBlock = ROWS * T;
@@ -744,8 +764,8 @@ __device__ __forceinline__ void row_item(const InstDev& P, typename Cx<R>::T* __
   } else {
 #pragma unroll
     for (int j = 0; j < E; ++j) {
-      r0[t + j * T] = a0[j];
-      r1[t + j * T] = a1[j];
+      st_global(r0 + t + j * T, a0[j]);
+      st_global(r1 + t + j * T, a1[j]);
     }
   }
 }
@@ -842,7 +862,7 @@ __global__ __launch_bounds__(ROWS * 2 * (N / E)) void row_split_kernel(const Ins
     for (int j = 0; j < E; ++j) a[j] = ld_stream(row + t + j * T);
   } else {
 #pragma unroll
-    for (int j = 0; j < E; ++j) a[j] = row[t + j * T];
+    for (int j = 0; j < E; ++j) a[j] = ld_global(row + t + j * T);
   }
   const double* vd0 = P.v_dense[0];
   const double* vd1 = P.v_dense[1];
@@ -913,7 +933,7 @@ __global__ __launch_bounds__(ROWS * 2 * (N / E)) void row_split_kernel(const Ins
     for (int j = 0; j < E; ++j) st_stream(row + t + j * T, a[j]);
   } else {
 #pragma unroll
-    for (int j = 0; j < E; ++j) row[t + j * T] = a[j];
+    for (int j = 0; j < E; ++j) st_global(row + t + j * T, a[j]);
   }
 }
 
@@ -1016,11 +1036,11 @@ __global__ __launch_bounds__(W*(N / E)) void energy_col_kernel(const InstDev* __
   const C* tw = stage_twiddles<tw_in_lds(kData, N, sizeof(C))>(tws, N, reinterpret_cast<C*>(smem) + W * N);
   C a[E];
 #pragma unroll
-  for (int j = 0; j < E; ++j) a[j] = col[(size_t)(t + j * T) * nx];
+  for (int j = 0; j < E; ++j) a[j] = ld_global(col + (size_t)(t + j * T) * nx);
   if constexpr (FWD) {
     fft_run<N, E, false, V>(a, t, tw, lds);
 #pragma unroll
-    for (int j = 0; j < E; ++j) col[(size_t)(t + j * T) * nx] = a[j];
+    for (int j = 0; j < E; ++j) st_global(col + (size_t)(t + j * T) * nx, a[j]);
     return;
   }
   const double* td = P.t_dense[comp];
@@ -1036,7 +1056,7 @@ __global__ __launch_bounds__(W*(N / E)) void energy_col_kernel(const InstDev* __
   }
   fft_run<N, E, true, V>(a, t, tw, lds);
 #pragma unroll
-  for (int j = 0; j < E; ++j) col[(size_t)(t + j * T) * nx] = a[j];
+  for (int j = 0; j < E; ++j) st_global(col + (size_t)(t + j * T) * nx, a[j]);
   double v[2] = {kin, s};
   block_sum<2>(v, smem);
   if (threadIdx.x == 0) {
@@ -1071,8 +1091,8 @@ __global__ __launch_bounds__(ROWS*(N / E)) void energy_row_kernel(const InstDev*
   C a0[E], a1[E];
 #pragma unroll
   for (int j = 0; j < E; ++j) {
-    a0[j] = r0[t + j * T];
-    a1[j] = r1[t + j * T];
+    a0[j] = ld_global(r0 + t + j * T);
+    a1[j] = ld_global(r1 + t + j * T);
   }
   fft_run2<N, E, true, V>(a0, a1, t, tw, lds0, lds1);
   const double* vd0 = P.v_dense[0];
@@ -1106,8 +1126,8 @@ __global__ __launch_bounds__(ROWS*(N / E)) void energy_row_kernel(const InstDev*
   const R back = (R)(1.0 / ((double)N * (double)ny));
 #pragma unroll
   for (int j = 0; j < E; ++j) {
-    r0[t + j * T] = cscale(a0[j], back);
-    r1[t + j * T] = cscale(a1[j], back);
+    st_global(r0 + t + j * T, cscale(a0[j], back));
+    st_global(r1 + t + j * T, cscale(a1[j], back));
   }
   double v[4] = {pot, inter, cpl, nrm};
   block_sum<4>(v, smem);
