@@ -25,6 +25,26 @@ template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
 template <typename C, typename R> __device__ __forceinline__ C cscale(C a, R s) { return C{a.x * s, a.y * s}; }
 template <typename C> __device__ __forceinline__ C cconj(C a) { return C{a.x, -a.y}; }
 
+// Loads and stores of one complex value through the global address space
+// (global_load / global_store, counted by vmcnt only). The passes' state and
+// table pointers come from slot descriptors (or a select with one), which the
+// compiler would access with flat instructions, and flat accesses count
+// against lgkmcnt as well: every later wait for a scalar load or for LDS
+// traffic then also waited for the state loads to arrive (or for the
+// stores to complete).
+template <typename C> __device__ __forceinline__ C ld_global(const C* p) {
+  typedef decltype(p->x) S;
+  typedef S V2 __attribute__((ext_vector_type(2)));
+  const V2 w = *(const __attribute__((address_space(1))) V2*)p;
+  return C{w.x, w.y};
+}
+__device__ __forceinline__ double ld_global(const double* p) { return *(const __attribute__((address_space(1))) double*)p; }
+template <typename C> __device__ __forceinline__ void st_global(C* p, C v) {
+  typedef decltype(v.x) S;
+  typedef S V2 __attribute__((ext_vector_type(2)));
+  *(__attribute__((address_space(1))) V2*)p = V2{v.x, v.y};
+}
+
 // Inputs and outputs of the standalone 2D transforms: up to two arrays per
 // launch (the two spinor components), passed by value as a kernel argument.
 struct FftPair {

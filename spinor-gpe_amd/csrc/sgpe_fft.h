@@ -322,7 +322,7 @@ template <int K, typename C> struct TableRegs {
   TableRegs<K - 1, C> rest;
   template <int N, int BLOCK> __device__ __forceinline__ void load(const C* __restrict__ s, int i) {
     if (N % BLOCK == 0 || i < N) {
-      const C v = s[i];
+      const C v = ld_global(s + i);
       re = v.x;
       im = v.y;
     }

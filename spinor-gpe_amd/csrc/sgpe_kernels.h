@@ -131,25 +131,6 @@ template <bool IMAG, typename C> __device__ __forceinline__ C mul_phase(C p, C u
 // variants: inside one loop the compiler would evaluate the sin/cos or exp
 // of every alternative for every point and select afterwards.
 
-// Loads and stores of one complex value through the global address space
-// (global_load / global_store, counted by vmcnt only). The passes' state and
-// table pointers come from slot descriptors (or a select with one), which the
-// compiler would access with flat instructions, and flat accesses count
-// against lgkmcnt as well: every later wait for a scalar load or for LDS
-// traffic then also waited for the state loads to arrive (or for the
-// stores to complete).
-template <typename C> __device__ __forceinline__ C ld_global(const C* p) {
-  typedef decltype(p->x) S;
-  typedef S V2 __attribute__((ext_vector_type(2)));
-  const V2 w = *(const __attribute__((address_space(1))) V2*)p;
-  return C{w.x, w.y};
-}
-template <typename C> __device__ __forceinline__ void st_global(C* p, C v) {
-  typedef decltype(v.x) S;
-  typedef S V2 __attribute__((ext_vector_type(2)));
-  *(__attribute__((address_space(1))) V2*)p = V2{v.x, v.y};
-}
-
 // The same with a non-temporal hint: every pass reads and writes each state
 // element exactly once, and streaming them past the caches shortens a pass
 // when a single instance leaves the chip mostly idle (for batched plans the
@@ -227,7 +208,7 @@ __device__ __forceinline__ double wave_sum_strided(const double* __restrict__ p,
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int ik = i + 64 * k;
-      e[k] = ik < n ? (COH ? ld_coherent(p + ik) : p[ik]) : 0.0;
+      e[k] = ik < n ? (COH ? ld_coherent(p + ik) : ld_global(p + ik)) : 0.0;
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k)
@@ -351,8 +332,8 @@ __device__ __forceinline__ void col_item(const InstDev& P, typename Cx<R>::T* __
   TableStage<kKy, N, kBlock, C> sp, sc_;
   TableStage<kTw, N, kBlock, C> stw;
   if (!td) {
-    kxp = reinterpret_cast<const C*>(P.kx[d_prev][comp])[jx];
-    kxc = reinterpret_cast<const C*>(P.kx[d_cur][comp])[jx];
+    kxp = ld_global(reinterpret_cast<const C*>(P.kx[d_prev][comp]) + jx);
+    kxc = ld_global(reinterpret_cast<const C*>(P.kx[d_cur][comp]) + jx);
   }
   // The y-tables are staged unconditionally (a conditional fill puts the
   // staging registers in scratch); without them (dense kinetic operator) the
@@ -673,11 +654,11 @@ __device__ __forceinline__ void row_item(const InstDev& P, typename Cx<R>::T* __
   const C* px0 = nullptr;
   const C* px1 = nullptr;
   if (!vd0) {
-    py0 = reinterpret_cast<const C*>(P.py[dur][0])[y];
+    py0 = ld_global(reinterpret_cast<const C*>(P.py[dur][0]) + y);
     px0 = reinterpret_cast<const C*>(P.px[dur][0]);
   }
   if (!vd1) {
-    py1 = reinterpret_cast<const C*>(P.py[dur][1])[y];
+    py1 = ld_global(reinterpret_cast<const C*>(P.py[dur][1]) + y);
     px1 = reinterpret_cast<const C*>(P.px[dur][1]);
   }
   const C* eth = reinterpret_cast<const C*>(P.etheta);
@@ -870,11 +851,11 @@ __global__ __launch_bounds__(ROWS * 2 * (N / E)) void row_split_kernel(const Ins
   const C* px0 = nullptr;
   const C* px1 = nullptr;
   if (!vd0) {
-    py0 = reinterpret_cast<const C*>(P.py[dur][0])[y];
+    py0 = ld_global(reinterpret_cast<const C*>(P.py[dur][0]) + y);
     px0 = reinterpret_cast<const C*>(P.px[dur][0]);
   }
   if (!vd1) {
-    py1 = reinterpret_cast<const C*>(P.py[dur][1])[y];
+    py1 = ld_global(reinterpret_cast<const C*>(P.py[dur][1]) + y);
     px1 = reinterpret_cast<const C*>(P.px[dur][1]);
   }
   const C* eth = reinterpret_cast<const C*>(P.etheta);
