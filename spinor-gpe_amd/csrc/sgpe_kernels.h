@@ -246,7 +246,8 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
 // round trips (≈1 µs of a 512-point column pass). The sums run over the
 // points in the same order either way. DENSE reads T in each loop (the second
 // read hits the cache) rather than keeping E values live across both.
-template <typename R, int E, int T, bool DENSE, bool IMAG, typename C>
+// KYG: the y tables are in global memory (else staged in LDS).
+template <typename R, int E, int T, bool DENSE, bool IMAG, bool KYG, typename C>
 __device__ __forceinline__ void col_kinetic(C (&a)[E], int t, int nx, int jx, const double* __restrict__ td, R hp,
                                             R hc, C kxp, C kxc, const C* kyp, const C* kyc, bool kspace, bool mid,
                                             double& s_acc, double& n_acc) {
@@ -255,9 +256,9 @@ __device__ __forceinline__ void col_kinetic(C (&a)[E], int t, int nx, int jx, co
     for (int j = 0; j < E; ++j) {
       C k;
       if constexpr (DENSE)
-        k = phase_op<R, IMAG>((R)td[(size_t)(t + j * T) * nx + jx] * hp);
+        k = phase_op<R, IMAG>((R)ld_global(td + (size_t)(t + j * T) * nx + jx) * hp);
       else
-        k = cmul(kxp, kyp[t + j * T]);
+        k = cmul(kxp, KYG ? ld_global(kyp + t + j * T) : kyp[t + j * T]);
       a[j] = cmul(a[j], k);
     }
   }
@@ -268,9 +269,9 @@ __device__ __forceinline__ void col_kinetic(C (&a)[E], int t, int nx, int jx, co
     for (int j = 0; j < E; ++j) {
       C k;
       if constexpr (DENSE)
-        k = phase_op<R, IMAG>((R)td[(size_t)(t + j * T) * nx + jx] * hc);
+        k = phase_op<R, IMAG>((R)ld_global(td + (size_t)(t + j * T) * nx + jx) * hc);
       else
-        k = cmul(kxc, kyc[t + j * T]);
+        k = cmul(kxc, KYG ? ld_global(kyc + t + j * T) : kyc[t + j * T]);
       a[j] = cmul(a[j], k);
       n_acc += norm2d<R>(a[j]);
     }
@@ -360,12 +361,14 @@ __device__ __forceinline__ void col_item(const InstDev& P, typename Cx<R>::T* __
   const R hc = (R)(0.5 * P.dt_sub[d_cur]);
   const bool mid = mode == COL_MID;
   if (!DENSE || !td)
-    col_kinetic<R, E, T, false, false>(a, t, nx, jx, nullptr, hp, hc, kxp, kxc, kyp, kyc, kspace, mid, s_acc, n_acc);
+    col_kinetic<R, E, T, false, false, !kKy>(a, t, nx, jx, nullptr, hp, hc, kxp, kxc, kyp, kyc, kspace, mid, s_acc,
+                                           n_acc);
   else if constexpr (DENSE) {
     if (P.imag)
-      col_kinetic<R, E, T, true, true>(a, t, nx, jx, td, hp, hc, kxp, kxc, kyp, kyc, kspace, mid, s_acc, n_acc);
+      col_kinetic<R, E, T, true, true, !kKy>(a, t, nx, jx, td, hp, hc, kxp, kxc, kyp, kyc, kspace, mid, s_acc, n_acc);
     else
-      col_kinetic<R, E, T, true, false>(a, t, nx, jx, td, hp, hc, kxp, kxc, kyp, kyc, kspace, mid, s_acc, n_acc);
+      col_kinetic<R, E, T, true, false, !kKy>(a, t, nx, jx, td, hp, hc, kxp, kxc, kyp, kyc, kspace, mid, s_acc,
+                                              n_acc);
   }
   if (mode == COL_MID) fft_run<N, E, true, V, kWave>(a, t, tw, lds);
   if constexpr ((CM & CM_STORE) != 0) {
