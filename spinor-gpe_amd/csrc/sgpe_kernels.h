@@ -1201,7 +1201,7 @@ __global__ __launch_bounds__(ROWS*(N / E)) void fft_rows_kernel(FftPair io, int 
   const C* tw = stage_twiddles<tw_in_lds(kData, N, sizeof(C))>(tws, N, reinterpret_cast<C*>(smem) + ROWS * N);
   C a[E];
 #pragma unroll
-  for (int j = 0; j < E; ++j) a[j] = src[t + j * T];
+  for (int j = 0; j < E; ++j) a[j] = ld_global(src + t + j * T);
   // (-1)^(x+y) with x = t + j*T: T is even, so the sign depends on t and y only.
   const R f = (R)(shift && ((t + y) & 1) ? -scale : scale);
   if (!inv) {
@@ -1214,7 +1214,7 @@ __global__ __launch_bounds__(ROWS*(N / E)) void fft_rows_kernel(FftPair io, int 
     for (int j = 0; j < E; ++j) a[j] = cscale(a[j], f);
   }
 #pragma unroll
-  for (int j = 0; j < E; ++j) dst[t + j * T] = a[j];
+  for (int j = 0; j < E; ++j) st_global(dst + t + j * T, a[j]);
 }
 
 template <typename R, int N, int E, int W>
@@ -1234,13 +1234,13 @@ __global__ __launch_bounds__(W*(N / E)) void fft_cols_kernel(FftPair io, int nx,
   const C* tw = stage_twiddles<tw_in_lds(kData, N, sizeof(C))>(tws, N, reinterpret_cast<C*>(smem) + W * N);
   C a[E];
 #pragma unroll
-  for (int j = 0; j < E; ++j) a[j] = in[(size_t)(t + j * T) * nx + jx];
+  for (int j = 0; j < E; ++j) a[j] = ld_global(in + (size_t)(t + j * T) * nx + jx);
   if (inv)
     fft_run<N, E, true, V>(a, t, tw, lds);
   else
     fft_run<N, E, false, V>(a, t, tw, lds);
 #pragma unroll
-  for (int j = 0; j < E; ++j) out[(size_t)(t + j * T) * nx + jx] = a[j];
+  for (int j = 0; j < E; ++j) st_global(out + (size_t)(t + j * T) * nx + jx, a[j]);
 }
 
 // Per-block sums of |psi0|^2 and |psi1|^2 (float64), written to part[2*block + s].
