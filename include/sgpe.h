@@ -103,8 +103,10 @@ void sgpe_ctx_destroy(sgpe_ctx* ctx);
  * n_arrays device arrays of shape (ny, nx):
  *   inverse = 0: out = fftshift(fftn(in)) * dx*dy/(2 pi)
  *   inverse = 1: out = ifftn(ifftshift(in)) / (dx*dy/(2 pi))
- * in[i] may equal out[i]; otherwise the arrays must not overlap (arrays are
- * transformed two per launch). Grid lengths as for sgpe_plan_create. */
+ * in[i] may equal out[i]; otherwise out[i] must not overlap in[i], and
+ * neither output of a launch pair (arrays 2m, 2m + 1 are transformed in one
+ * launch) may overlap the other array's input or output: SGPE_E_ARG if so.
+ * Grid lengths as for sgpe_plan_create. */
 int sgpe_fft2(sgpe_ctx* ctx, int precision, int ny, int nx, int n_arrays, void* const* in, void* const* out,
               int inverse, double dx, double dy);
 

@@ -1169,6 +1169,20 @@ int sgpe_fft2(sgpe_ctx* ctx, int precision, int ny, int nx, int n_arrays, void* 
     };
     for (int i = 0; i < n_arrays; ++i)
       if (!in[i] || !out[i]) return fail(SGPE_E_ARG, "NULL array");
+    // Byte ranges [p, p + bytes) of a and b overlap.
+    const size_t bytes = (size_t)ny * nx * cs;
+    auto overlap = [bytes](const void* a, const void* b) {
+      const char *pa = (const char*)a, *pb = (const char*)b;
+      return pa < pb + bytes && pb < pa + bytes;
+    };
+    for (int i = 0; i < n_arrays; ++i) {
+      if (in[i] != out[i] && overlap(in[i], out[i]))
+        return fail(SGPE_E_ARG, "an output partly overlaps its own input");
+      // Arrays 2m and 2m + 1 are transformed in the same launch.
+      const int j = i ^ 1;
+      if (j < n_arrays && (overlap(out[i], out[j]) || overlap(out[i], in[j])))
+        return fail(SGPE_E_ARG, "an output overlaps another array of its launch pair");
+    }
     for (int i = 0; i < n_arrays; i += 2) {
       const int k = std::min(2, n_arrays - i);
       FftPair src{}, dst{};

@@ -44,6 +44,37 @@ def test_fft2_matches_numpy(gpu, shape, dtype):
     assert rel(back, back_want) < TOL[dtype] * logn
 
 
+def test_fft2_odd_count_in_place_and_overlap_checks(gpu):
+    """Three arrays (one launch pair plus a single), in place and out of
+    place, against numpy; outputs that overlap an array of their launch pair
+    are refused before any launch."""
+    from spinor_gpe import _native
+    print("[fft2-pairs] shape=(128, 256) n=3", flush=True)
+    rng = np.random.default_rng(7)
+    shape = (128, 256)
+    psi = [rng.standard_normal(shape) + 1j * rng.standard_normal(shape) for _ in range(3)]
+    dr = (0.3, 0.2)
+    want = tt.fft_2d(psi, dr)
+    dev = [torch.as_tensor(p, device=gpu) for p in psi]
+    got = _native.fft2(dev, False, *dr)
+    assert rel([g.cpu().numpy() for g in got], want) < 4e-15 * 15
+    inplace = [d.clone() for d in dev]
+    _native.fft2(inplace, False, *dr, out=inplace)
+    assert rel([g.cpu().numpy() for g in inplace], want) < 4e-15 * 15
+    big = torch.zeros((2 * shape[0] + 1,) + shape[1:], dtype=torch.complex128, device=gpu)
+    shifted = big[1:shape[0] + 1]                     # overlaps both halves below
+    with pytest.raises(_native.SgpeError, match='launch pair'):
+        _native.fft2(dev[:2], False, *dr, out=[big[:shape[0]], shifted])
+    with pytest.raises(_native.SgpeError, match='launch pair'):
+        _native.fft2(dev[:2], False, *dr, out=[dev[1], big[:shape[0]]])
+    with pytest.raises(_native.SgpeError, match='its own input'):
+        _native.fft2([big[:shape[0]]], False, *dr, out=[shifted])
+    # Arrays of different pairs run in order and may overlap.
+    a = [d.clone() for d in dev]
+    _native.fft2(a, False, *dr, out=[a[0], a[1], a[0]])
+    assert rel([a[0].cpu().numpy()], [want[2]]) < 4e-15 * 15
+
+
 @pytest.mark.parametrize('shape', list(itertools.product([128, 256, 512, 1024], repeat=2)))
 def test_reference_fft_checks_on_ones(gpu, shape):
     """fft_func_tests.py:27-56, 421-472 on device tensors."""
