@@ -370,16 +370,22 @@ __device__ __forceinline__ void col_item(const InstDev& P, typename Cx<R>::T* __
       col_kinetic<R, E, T, true, false, !kKy>(a, t, nx, jx, td, hp, hc, kxp, kxc, kyp, kyc, kspace, mid, s_acc,
                                               n_acc);
   }
-  if (mode == COL_MID) fft_run<N, E, true, V, kWave>(a, t, tw, lds);
+  // Columns of 1024 points and more: an opaque copy of t for the inverse
+  // transform and the stores, so their addresses are recomputed rather than
+  // held since the loads (as in row_item; 2-5 % per pass there, while
+  // shorter batched columns lose 1-2 % to the recomputation).
+  int tf = t;
+  if constexpr (N >= 1024) asm volatile("" : "+v"(tf));
+  if (mode == COL_MID) fft_run<N, E, true, V, kWave>(a, tf, tw, lds);
   if constexpr ((CM & CM_STORE) != 0) {
 #pragma unroll
-    for (int j = 0; j < E; ++j) cb.store<C>(cbase + (uint32_t)((t + j * T) * nx * sizeof(C)), a[j]);
+    for (int j = 0; j < E; ++j) cb.store<C>(cbase + (uint32_t)((tf + j * T) * nx * sizeof(C)), a[j]);
   } else if (stream) {
 #pragma unroll
-    for (int j = 0; j < E; ++j) st_stream(col + (size_t)(t + j * T) * nx, a[j]);
+    for (int j = 0; j < E; ++j) st_stream(col + (size_t)(tf + j * T) * nx, a[j]);
   } else {
 #pragma unroll
-    for (int j = 0; j < E; ++j) st_global(col + (size_t)(t + j * T) * nx, a[j]);
+    for (int j = 0; j < E; ++j) st_global(col + (size_t)(tf + j * T) * nx, a[j]);
   }
 
   double v[2] = {s_acc, n_acc};
@@ -725,14 +731,29 @@ __device__ __forceinline__ void row_item(const InstDev& P, typename Cx<R>::T* __
         row_pointwise<R, N, E, T, false, !GENERAL, H, E>(a0, a1, P, dur, y, t, lam2, py0, py1, tx0, tx1, tex);
     }
   }
+  // Large rows: the forward transform and the stores index by an opaque copy
+  // of t, so their LDS and state addresses are recomputed here instead of
+  // being held in registers since the loads and the inverse transform, which
+  // compute the same addresses (held, they cost up to 70 VGPRs and spill the
+  // 4096-point complex128 pass). It pays where registers are what limits
+  // the resident workgroups: 4096 points, and 1024-2048 points where the
+  // LDS admits three waves per SIMD (complex64 2048 points: 192 -> 130
+  // VGPRs, 58 -> 50 us per pass). Where the LDS allows two waves anyway
+  // (complex128 2048 points) or small batched rows, whose workgroups fill
+  // the CUs' slots already, only the recomputation remains (+2 %).
+  constexpr size_t kLdsWg = kData * (kDb ? 2 : 1) + (kTw ? (size_t)N * sizeof(C) : 0);
+  constexpr int kLdsWaves = (int)(kLdsBytes / kLdsWg) * ((ROWS * T + 63) / 64) / 4;
+  constexpr bool kOpaque = N >= 4096 || (N >= 1024 && kLdsWaves >= 3);
+  int tf = t;
+  if constexpr (kOpaque) asm volatile("" : "+v"(tf));
   // The forward transform starts on the buffer pair the inverse one read
   // last but one, so no barrier is needed between them.
   if constexpr (kDb && fft_exchanges<N, E>() % 2)
-    fft_run2_db<N, E, false, V, kWave>(a0, a1, t, tw, alt0, alt1, lds0, lds1);
+    fft_run2_db<N, E, false, V, kWave>(a0, a1, tf, tw, alt0, alt1, lds0, lds1);
   else if constexpr (kDb)
-    fft_run2_db<N, E, false, V, kWave>(a0, a1, t, tw, lds0, lds1, alt0, alt1);
+    fft_run2_db<N, E, false, V, kWave>(a0, a1, tf, tw, lds0, lds1, alt0, alt1);
   else
-    fft_run2<N, E, false, V, kWave>(a0, a1, t, tw, lds0, lds1);
+    fft_run2<N, E, false, V, kWave>(a0, a1, tf, tw, lds0, lds1);
   if constexpr ((CM & CM_STORE) != 0) {
 #pragma unroll
     for (int j = 0; j < E; ++j) {
@@ -742,14 +763,14 @@ __device__ __forceinline__ void row_item(const InstDev& P, typename Cx<R>::T* __
   } else if (stream) {
 #pragma unroll
     for (int j = 0; j < E; ++j) {
-      st_stream(r0 + t + j * T, a0[j]);
-      st_stream(r1 + t + j * T, a1[j]);
+      st_stream(r0 + tf + j * T, a0[j]);
+      st_stream(r1 + tf + j * T, a1[j]);
     }
   } else {
 #pragma unroll
     for (int j = 0; j < E; ++j) {
-      st_global(r0 + t + j * T, a0[j]);
-      st_global(r1 + t + j * T, a1[j]);
+      st_global(r0 + tf + j * T, a0[j]);
+      st_global(r1 + tf + j * T, a1[j]);
     }
   }
 }
