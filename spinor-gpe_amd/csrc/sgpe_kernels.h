@@ -1234,8 +1234,12 @@ __global__ __launch_bounds__(ROWS*(N / E)) void fft_rows_kernel(FftPair io, int 
 #pragma unroll
     for (int j = 0; j < E; ++j) a[j] = cscale(a[j], f);
   }
+  // Store addresses from an opaque copy of t: recomputed here rather than
+  // held in registers since the loads (as in the pass kernels).
+  int tf = t;
+  asm volatile("" : "+v"(tf));
 #pragma unroll
-  for (int j = 0; j < E; ++j) st_global(dst + t + j * T, a[j]);
+  for (int j = 0; j < E; ++j) st_global(dst + tf + j * T, a[j]);
 }
 
 template <typename R, int N, int E, int W>
@@ -1260,8 +1264,12 @@ __global__ __launch_bounds__(W*(N / E)) void fft_cols_kernel(FftPair io, int nx,
     fft_run<N, E, true, V>(a, t, tw, lds);
   else
     fft_run<N, E, false, V>(a, t, tw, lds);
+  // Store addresses from an opaque copy of t (complex64 2048-point columns:
+  // 84 -> 60 VGPRs, room for a second 1024-thread workgroup per CU).
+  int tf = t;
+  asm volatile("" : "+v"(tf));
 #pragma unroll
-  for (int j = 0; j < E; ++j) st_global(out + (size_t)(t + j * T) * nx + jx, a[j]);
+  for (int j = 0; j < E; ++j) st_global(out + (size_t)(tf + j * T) * nx + jx, a[j]);
 }
 
 // Per-block sums of |psi0|^2 and |psi1|^2 (float64), written to part[2*block + s].
