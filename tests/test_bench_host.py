@@ -41,6 +41,24 @@ def test_shipped_shapes_have_traffic_entries(prec, n):
         assert 1.0 <= traffic / alg < 1.1, (name, traffic / alg)
 
 
+def test_shipped_table_names_compiled_shapes():
+    """Every entry of the shipped MI355X table selects a row shape and a
+    column shape the library compiles (negative counts: split rows, column
+    tiles with global y tables), for its own grid lengths."""
+    with open(autotune.PACKAGED) as f:
+        table = json.load(f)
+    assert table
+    for k, entry in table.items():
+        prec_code, grid, slots = k.split(':')
+        ny, nx = (int(v) for v in grid.split('x'))
+        prec = ('c128', 'c64')[int(prec_code)]
+        shape = entry['shape']
+        assert len(shape) == 6 and int(slots) >= 1, k
+        assert tuple(shape[0:2]) in autotune.shapes(0, prec, nx), (k, shape)
+        assert tuple(shape[2:4]) in autotune.shapes(1, prec, ny), (k, shape)
+        assert shape[4] in (0, 1) and shape[5] in (0, 1), (k, shape)
+
+
 def test_roofline_arithmetic():
     res = {'row_ms': 0.0125, 'col_ms': 0.010, 'tuning': (4, 2, 8, 4), 'streaming': (1, 1)}
     r = bench.roofline(res, 'c128', 512)
