@@ -62,7 +62,7 @@ def test_fft2_odd_count_in_place_and_overlap_checks(gpu):
     _native.fft2(inplace, False, *dr, out=inplace)
     assert rel([g.cpu().numpy() for g in inplace], want) < 4e-15 * 15
     big = torch.zeros((2 * shape[0] + 1,) + shape[1:], dtype=torch.complex128, device=gpu)
-    shifted = big[1:shape[0] + 1]                     # overlaps both halves below
+    shifted = big[1:shape[0] + 1]                     # partly overlaps big[:shape[0]]
     with pytest.raises(_native.SgpeError, match='launch pair'):
         _native.fft2(dev[:2], False, *dr, out=[big[:shape[0]], shifted])
     with pytest.raises(_native.SgpeError, match='launch pair'):
