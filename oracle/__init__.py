@@ -24,7 +24,9 @@ Gaussian known answers, Parseval, exact free-particle evolution, exact Rabi
 oscillations, the harmonic ground state, real-time norm conservation and
 time reversibility (which distinguishes the reference's reuse of the
 interaction operator from a recomputation), second-order step convergence
-and the virial identity. ``eng_expect`` on phase-wrapped states depends on
-scikit-image's unwrap_phase, which is absent here: parity unpinned for those
-states (wrap-free states are pinned).
+and the virial identity. ``eng_expect`` on states whose phase has residues
+(vortices) depends on scikit-image's unwrap_phase, which is absent here:
+parity unpinned for those states. Wrap-free states, and wrapped states
+without residues (a chirped phase, where every correct unwrap agrees up to
+a global 2 pi), are pinned.
 """

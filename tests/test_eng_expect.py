@@ -15,7 +15,12 @@ fixed by arithmetic that does not repeat its formula:
   dr[0] as the axis-0 spacing returns q dy/dx along y (resp. q dx/dy along x)
   exactly at every pixel, so the kinetic term is (1/2) N_pix sum_s n_s
   (q'^2 + 2 kL q') for the y-ramp (the recoil term is on the axis-0
-  derivative) and (1/2) N_pix sum_s n_s q'^2 for the x-ramp.
+  derivative) and (1/2) N_pix sum_s n_s q'^2 for the x-ramp;
+* a chirp exp(i (alpha y^2 + beta x^2)) that wraps several times along both
+  axes without residues: any correct unwrap returns the phase up to a global
+  2 pi multiple, and the finite differences of a square are known in closed
+  form, so the kinetic term is fixed; leaving the unwrap out, or unwrapping
+  along rows only, fails it.
 
 The planted-mistake tests run each check on the oracle with one plausible
 slip built in (a 1/2 on the interaction, the area element, the coupling phase
@@ -126,8 +131,46 @@ def check_ramp_x(fn):
     assert e[1] == pytest.approx(want, rel=1e-10)
 
 
+def fd_of_square(c, a, h):
+    """np.gradient (edge_order 1, spacing h) of a*c**2 sampled on the uniform
+    coordinates c, from the algebra of differences of squares: interior
+    (a c[i+1]^2 - a c[i-1]^2) / 2h = 2 a c[i] d / h with d the coordinate
+    step; first and last points a (c[0] + c[1]) d / h and
+    a (c[-2] + c[-1]) d / h."""
+    d = c[1] - c[0]
+    out = 2 * a * c * d / h
+    out[0] = a * (c[0] + c[1]) * d / h
+    out[-1] = a * (c[-2] + c[-1]) * d / h
+    return out
+
+
+def check_chirp(fn):
+    """Uniform moduli with the curved phase alpha y^2 + beta x^2, which wraps
+    several times along both axes (ranges ~19 and ~18 rad) with neighbour
+    steps below pi: the field has no residues, so every correct unwrapper
+    returns it up to one global multiple of 2 pi, which no gradient sees.
+    The kinetic term is then (1/2) sum_s n_s sum_pix (g0^2 + g1^2 + 2 kL g0)
+    with g0, g1 the finite differences of the two quadratics (axis 0 with
+    dr[0] = dx as its spacing)."""
+    ps = make()
+    dx, dy = ps.space['dr']
+    x, y = ps.space['x_mesh'], ps.space['y_mesh']
+    ny, nx = x.shape
+    alpha, beta, kl, b0, b1 = 0.3, 0.5, 1.7, 1.1, 0.7
+    phase = alpha * y ** 2 + beta * x ** 2
+    assert np.ptp(alpha * y[:, 0] ** 2) > 4 * np.pi and np.ptp(beta * x[0] ** 2) > 4 * np.pi
+    assert np.max(np.abs(np.diff(phase, axis=0))) < 0.9 * np.pi
+    assert np.max(np.abs(np.diff(phase, axis=1))) < 0.9 * np.pi
+    psi = [b0 * np.exp(1j * phase), b1 * np.exp(1j * (phase + 2.0))]
+    e = fn(ps, to_k(ps, psi), np.zeros_like(x), kl, True)
+    g0 = fd_of_square(y[:, 0], alpha, dx)[:, None] * np.ones((1, nx))
+    g1 = fd_of_square(x[0], beta, dy)[None, :] * np.ones((ny, 1))
+    want = 0.5 * (b0 ** 2 + b1 ** 2) * np.sum(g0 ** 2 + g1 ** 2 + 2 * kl * g0)
+    assert e[1] == pytest.approx(want, rel=1e-9)
+
+
 CHECKS = {'uniform': check_uniform, 'coupling_pair': check_coupling_pair,
-          'ramp_y': check_ramp_y, 'ramp_x': check_ramp_x}
+          'ramp_y': check_ramp_y, 'ramp_x': check_ramp_x, 'chirp': check_chirp}
 
 
 @pytest.mark.parametrize('impl', sorted(IMPLS))
@@ -137,7 +180,7 @@ def test_eng_expect_closed_form(impl, check):
 
 
 PLANT_CAUGHT_BY = {'half_int': ['uniform'], 'area': ['uniform'],
-                   'cpl_theta': ['coupling_pair'], 'recoil_axis1': ['ramp_y', 'ramp_x']}
+                   'cpl_theta': ['coupling_pair'], 'recoil_axis1': ['ramp_y', 'ramp_x', 'chirp']}
 
 
 @pytest.mark.parametrize('plant', oe.PLANTS)
@@ -147,6 +190,20 @@ def test_planted_mistakes_fail(plant):
     for name in PLANT_CAUGHT_BY[plant]:
         with pytest.raises(AssertionError):
             CHECKS[name](planted)
+
+
+@pytest.mark.parametrize('unwrap', ['none', 'rows_only'])
+def test_chirp_catches_a_missing_or_partial_unwrap(unwrap):
+    """The chirp check fails for the oracle with the unwrap left out, or
+    done along rows only (numpy's 1-D unwrap on axis 1), so it pins the
+    two-dimensional unwrap the reference's functional relies on."""
+    fn = None if unwrap == 'none' else (lambda a: np.unwrap(a, axis=1))
+
+    def variant(ps, psik, coupling, kl, is_coupling):
+        return oe.eng_expect(psik, ps.space['dr'], ps.pot_eng_spin, coupling, ps.g_sc, kl, is_coupling,
+                             unwrap=fn, theta=2 * kl * ps.space['x_mesh'])
+    with pytest.raises(AssertionError):
+        check_chirp(variant)
 
 
 def test_engine_and_oracle_agree_on_a_noisy_state():
