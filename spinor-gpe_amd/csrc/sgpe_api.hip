@@ -1169,10 +1169,11 @@ int sgpe_fft2(sgpe_ctx* ctx, int precision, int ny, int nx, int n_arrays, void* 
     };
     for (int i = 0; i < n_arrays; ++i)
       if (!in[i] || !out[i]) return fail(SGPE_E_ARG, "NULL array");
-    // Byte ranges [p, p + bytes) of a and b overlap.
-    const size_t bytes = (size_t)ny * nx * cs;
+    // Byte ranges [p, p + bytes) of a and b overlap (compared as integers:
+    // the arrays may belong to unrelated allocations).
+    const uintptr_t bytes = (uintptr_t)ny * nx * cs;
     auto overlap = [bytes](const void* a, const void* b) {
-      const char *pa = (const char*)a, *pb = (const char*)b;
+      const uintptr_t pa = (uintptr_t)a, pb = (uintptr_t)b;
       return pa < pb + bytes && pb < pa + bytes;
     };
     for (int i = 0; i < n_arrays; ++i) {
