@@ -135,7 +135,7 @@ static void default_col(int prec, int n, int* e, int* w) {
     case 256: *e = 8; *w = 4; break;
     case 512: *e = 8; *w = 4; break;
     case 1024: *e = 8; *w = 4; break;
-    case 2048: *e = 8; *w = 4; break;
+    case 2048: *e = prec == SGPE_C128 ? 8 : 16; *w = 4; break;
     default: *e = prec == SGPE_C128 ? 8 : 16; *w = prec == SGPE_C128 ? 2 : 4; break;
   }
 }

@@ -202,7 +202,7 @@ void erow_launch(dim3 g, dim3 b, size_t l, hipStream_t s, const InstDev* i, cons
   X(R, PR, 2048, 8, 4) X(R, PR, 4096, 8, 2)
 #define SGPE_DEF_COLS_F32(X, R, PR)                                                                 \
   X(R, PR, 64, 8, 8) X(R, PR, 128, 8, 8) X(R, PR, 256, 8, 4) X(R, PR, 512, 8, 4) X(R, PR, 1024, 8, 4) \
-  X(R, PR, 2048, 8, 4) X(R, PR, 4096, 16, 4)
+  X(R, PR, 2048, 16, 4) X(R, PR, 4096, 16, 4)
 
 #define SGPE_DEFINE_REGISTRY(FN, R, PR, DEFCOLS, COLONLY)                                            \
   namespace sgpe {                                                                                   \
