@@ -1031,7 +1031,7 @@ __global__ __launch_bounds__(W*(N / E)) void energy_col_kernel(const InstDev* __
   const InstDev& P = insts[inst];
   const int nx = P.nx;
   const int comp = blockIdx.y;
-  const int tile = blockIdx.x;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);  // neighbouring tiles share an L2
   const int c = threadIdx.x % W;
   const int t = threadIdx.x / W;
   const int jx = tile * W + c;
@@ -1253,7 +1253,7 @@ __global__ __launch_bounds__(W*(N / E)) void fft_cols_kernel(FftPair io, int nx,
   C* out = reinterpret_cast<C*>(io.out[blockIdx.z]);
   const int c = threadIdx.x % W;
   const int t = threadIdx.x / W;
-  const int jx = blockIdx.x * W + c;
+  const int jx = xcd_tile(blockIdx.x, gridDim.x) * W + c;  // neighbouring tiles share an L2
   C* lds = reinterpret_cast<C*>(smem) + c;
   constexpr size_t kData = (size_t)W * N * sizeof(C);
   const C* tw = stage_twiddles<tw_in_lds(kData, N, sizeof(C))>(tws, N, reinterpret_cast<C*>(smem) + W * N);
