@@ -59,6 +59,17 @@ def test_shipped_table_names_compiled_shapes():
         assert shape[4] in (0, 1) and shape[5] in (0, 1), (k, shape)
 
 
+def test_more_than_one_gpu_is_refused():
+    """``bench.py --gpus N`` with N > 1 exits (status 2) with a message saying
+    that the engine measures one GPU, before any GPU work."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, 'bench.py'), '--gpus', '2'],
+                       capture_output=True, text=True, timeout=300, cwd=ROOT)
+    assert r.returncode == 2
+    assert 'measures one GPU' in r.stderr + r.stdout
+
+
 def test_roofline_arithmetic():
     res = {'row_ms': 0.0125, 'col_ms': 0.010, 'tuning': (4, 2, 8, 4), 'streaming': (1, 1)}
     r = bench.roofline(res, 'c128', 512)
